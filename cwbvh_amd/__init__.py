@@ -1,0 +1,655 @@
+"""cwbvh_amd: MI355X (gfx950) ray traversal -- CWBVH and BVH2, closest and any hit.
+
+Python host side over the C ABI of ``libcwbvh_amd.so`` (include/cwbvh_amd.h).
+The library is built in-tree by ``python -m cwbvh_amd.build``; importing this
+package without it raises ImportError -- there is no fallback path.
+
+Mirrors the reference's host surface:
+  * ``rtBindCWBVHData`` / ``rtTraceCWBVH`` and ``rtBindBVH2Data`` / ``rtTraceBVH2``
+    (reference src/TraversalKernelCWBVH.h:3-11, src/TraversalKernelBVH2.h:3-11);
+  * scene + ray generators standing in for src/main.cpp:29-86 and
+    src/ValidationKernels.cu;
+  * ``build_layouts`` standing in for the Embree build + GPUBVHConverter
+    (src/main.cpp:88-200).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+_LIB_PATH = Path(__file__).resolve().with_name("libcwbvh_amd.so")
+
+KIND_BVH2 = 0
+KIND_CWBVH = 1
+TRACE_CLOSEST = 0
+TRACE_ANY = 1
+TRACE_PRIM_ID = 2
+TRACE_STATS = 4
+STACK_CAPACITY = 64
+BVH2_STACK_CAPACITY = 64
+
+RAY_DTYPE = np.float32  # rays: (n, 8) float32 = origin.xyz, tmin, dir.xyz, tmax
+HIT_DTYPE = np.float32  # hits: (n, 4) float32 = t, triAddr bits, u, v
+
+
+class CwbvhError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"cwbvh status {status}: {message}")
+        self.status = status
+
+
+class BuildParams(C.Structure):
+    _fields_ = [
+        ("num_threads", C.c_int32),
+        ("sah_bins", C.c_int32),
+        ("bvh2_node_cost", C.c_float),
+        ("bvh2_tri_cost", C.c_float),
+        ("bvh2_max_leaf", C.c_int32),
+        ("cwbvh_node_cost", C.c_float),
+        ("cwbvh_tri_cost", C.c_float),
+        ("cwbvh_node_stride", C.c_int32),
+        ("max_depth", C.c_int32),
+    ]
+
+
+class LayoutView(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("node_stride", C.c_int32),
+        ("nodes", C.c_void_p),
+        ("nodes_f4", C.c_int64),
+        ("woop", C.c_void_p),
+        ("woop_f4", C.c_int64),
+        ("prim_idx", C.c_void_p),
+        ("prim_idx_len", C.c_int64),
+        ("max_depth", C.c_int32),
+        ("stack_need", C.c_int32),
+        ("num_nodes", C.c_int64),
+        ("num_leaves", C.c_int64),
+        ("num_tri_refs", C.c_int64),
+        ("sah_cost", C.c_double),
+    ]
+
+
+class TraceConfig(C.Structure):
+    _fields_ = [
+        ("block_size", C.c_int32),
+        ("lds_depth", C.c_int32),
+        ("grid_blocks", C.c_int32),
+        ("fetch_chunk", C.c_int32),
+        ("refill_idle", C.c_int32),
+        ("refill_budget", C.c_int32),
+        ("postpone_pct", C.c_int32),
+        ("prefetch", C.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [
+        ("name", C.c_char * 64),
+        ("arch", C.c_char * 32),
+        ("compute_units", C.c_int32),
+        ("wave_size", C.c_int32),
+        ("lds_per_block", C.c_int64),
+        ("l2_bytes", C.c_int64),
+        ("global_mem_bytes", C.c_int64),
+        ("clock_khz", C.c_int32),
+        ("mem_clock_khz", C.c_int32),
+        ("mem_bus_width", C.c_int32),
+    ]
+
+
+_P = C.c_void_p
+_I32 = C.c_int32
+_I64 = C.c_int64
+_U32 = C.c_uint32
+_U64 = C.c_uint64
+_F = C.c_float
+_PP = C.POINTER(C.c_void_p)
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "cwbvh_abi_version": (_I32, []),
+    "cwbvh_status_string": (C.c_char_p, [_I32]),
+    "cwbvh_last_error": (C.c_char_p, []),
+    "cwbvh_mesh_from_arrays": (_I32, [_P, _I64, _P, _I64, _PP]),
+    "cwbvh_mesh_view": (_I32, [_P, _PP, C.POINTER(_I64), _PP, C.POINTER(_I64)]),
+    "cwbvh_mesh_bounds": (_I32, [_P, _P, _P]),
+    "cwbvh_mesh_free": (None, [_P]),
+    "cwbvh_mesh_heightfield": (_I32, [_I32, _I32, _I32, _F, _U64, _PP]),
+    "cwbvh_mesh_icosphere_grid": (_I32, [_I32, _I32, _I32, _I32, _F, _F, _F, _I32, _U64, _PP]),
+    "cwbvh_mesh_cluster_soup": (_I32, [_I64, _I32, _F, _F, _F, _F, _U64, _PP]),
+    "cwbvh_mesh_validation_planes": (_I32, [_I32, _PP]),
+    "cwbvh_rays_ao_surface": (_I32, [_P, _I64, _I32, _F, _F, _F, _U64, _P]),
+    "cwbvh_rays_pinhole": (_I32, [_I32, _I32, _P, _P, _P, _F, _F, _F, _P]),
+    "cwbvh_rays_ao_from_hits": (_I32, [_P, _P, _P, _P, _I64, _I32, _F, _F, _U64, _P, C.POINTER(_I64)]),
+    "cwbvh_rays_sphere_from_surface": (_I32, [_P, _I64, _F, _F, _F, _U64, _P]),
+    "cwbvh_rays_validation": (_I32, [_I64, _I32, _P]),
+    "cwbvh_rays_shuffle": (_I32, [_P, _I64, _U64]),
+    "cwbvh_build_params_default": (_I32, [_P]),
+    "cwbvh_build_layouts": (_I32, [_P, _P, _PP, _PP]),
+    "cwbvh_layout_from_arrays": (_I32, [_I32, _I32, _P, _I64, _P, _I64, _P, _I64, _PP]),
+    "cwbvh_layout_get_view": (_I32, [_P, _P]),
+    "cwbvh_layout_restride": (_I32, [_P, _I32, _PP]),
+    "cwbvh_layout_free": (None, [_P]),
+    "cwbvh_woopify": (_I32, [_P, _P, _P, _P]),
+    "cwbvh_device_count": (_I32, [C.POINTER(_I32)]),
+    "cwbvh_get_device_info": (_I32, [_I32, _P]),
+    "cwbvh_malloc": (_I32, [_PP, _I64]),
+    "cwbvh_free": (_I32, [_P]),
+    "cwbvh_memcpy_htod": (_I32, [_P, _P, _I64]),
+    "cwbvh_memcpy_dtoh": (_I32, [_P, _P, _I64]),
+    "cwbvh_memcpy_dtod": (_I32, [_P, _P, _I64]),
+    "cwbvh_memset": (_I32, [_P, _I32, _I64]),
+    "cwbvh_stream_create": (_I32, [_PP]),
+    "cwbvh_stream_destroy": (_I32, [_P]),
+    "cwbvh_stream_synchronize": (_I32, [_P]),
+    "cwbvh_device_synchronize": (_I32, []),
+    "cwbvh_event_create": (_I32, [_PP]),
+    "cwbvh_event_destroy": (_I32, [_P]),
+    "cwbvh_event_record": (_I32, [_P, _P]),
+    "cwbvh_event_synchronize": (_I32, [_P]),
+    "cwbvh_event_elapsed_ms": (_I32, [_P, _P, C.POINTER(_F)]),
+    "cwbvh_copy_u4": (_I32, [_P, _P, _I64, _P]),
+    "cwbvh_scene_create": (_I32, [_P, _P, _PP]),
+    "cwbvh_scene_wrap_device": (_I32, [_I32, _I32, _P, _P, _P, _I32, _PP]),
+    "cwbvh_scene_device_bytes": (_I32, [_P, C.POINTER(_I64)]),
+    "cwbvh_scene_kind": (_I32, [_P, C.POINTER(_I32)]),
+    "cwbvh_scene_destroy": (None, [_P]),
+    "cwbvh_trace_config_default": (_I32, [_I32, _P]),
+    "cwbvh_trace": (_I32, [_P, _P, _P, _I64, _U32, _P, _P, _P]),
+    "cwbvh_trace_ordered": (_I32, [_P, _P, _P, _P, _I64, _U32, _P, _P, _P]),
+    "cwbvh_stack_overflowed": (_I32, [_P, C.POINTER(_I32)]),
+    "rtBindCWBVHData": (None, [_P, _P, _P]),
+    "rtTraceCWBVH": (None, [_P, _P, C.c_int]),
+    "rtBindBVH2Data": (None, [_P, _P, _P]),
+    "rtTraceBVH2": (None, [_P, _P, C.c_int]),
+    "cwbvh_last_status": (_I32, []),
+    "cwbvh_last_trace_ms": (_F, []),
+    "cwbvh_set_log": (None, [_I32]),
+}
+
+
+def _load() -> C.CDLL:
+    if not _LIB_PATH.exists():
+        raise ImportError(f"{_LIB_PATH} is missing; build it with `python -m cwbvh_amd.build`")
+    lib = C.CDLL(str(_LIB_PATH))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+LIB_PATH = str(_LIB_PATH)
+EXPORTED_SYMBOLS = tuple(_SIGS)
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise CwbvhError(status, f"{lib.cwbvh_status_string(status).decode()}: {lib.cwbvh_last_error().decode()}")
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+# ------------------------------------------------------------------ meshes
+
+
+class Mesh:
+    """Host triangle mesh (vertices float32 (nv,3), indices int32 (nt,3))."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.cwbvh_mesh_free(h)
+            self._h = C.c_void_p(0)
+
+    @staticmethod
+    def _make(fn, *args) -> "Mesh":
+        out = C.c_void_p()
+        check(fn(*args, C.byref(out)))
+        return Mesh(out.value)
+
+    @classmethod
+    def from_arrays(cls, vertices, indices) -> "Mesh":
+        v = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        i = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1, 3)
+        return cls._make(lib.cwbvh_mesh_from_arrays, _ptr(v), len(v), _ptr(i), len(i))
+
+    @classmethod
+    def heightfield(cls, nx=256, ny=128, octaves=8, amp0=0.1, seed=0) -> "Mesh":
+        return cls._make(lib.cwbvh_mesh_heightfield, nx, ny, octaves, amp0, seed)
+
+    @classmethod
+    def icosphere_grid(cls, gx=10, gy=10, gz=10, frequency=7, spacing=4.0, scale_min=0.5, scale_max=1.5,
+                       ground_plane=True, seed=0) -> "Mesh":
+        return cls._make(lib.cwbvh_mesh_icosphere_grid, gx, gy, gz, frequency, spacing, scale_min, scale_max,
+                         1 if ground_plane else 0, seed)
+
+    @classmethod
+    def cluster_soup(cls, num_triangles, num_clusters=4096, sigma_min=0.005, sigma_max=0.05,
+                     edge_mu=float(np.log(1e-3)), edge_sigma=0.5, seed=0) -> "Mesh":
+        return cls._make(lib.cwbvh_mesh_cluster_soup, num_triangles, num_clusters, sigma_min, sigma_max, edge_mu,
+                         edge_sigma, seed)
+
+    @classmethod
+    def validation_planes(cls, count) -> "Mesh":
+        return cls._make(lib.cwbvh_mesh_validation_planes, count)
+
+    def _view(self):
+        vp, ip = C.c_void_p(), C.c_void_p()
+        nv, nt = C.c_int64(), C.c_int64()
+        check(lib.cwbvh_mesh_view(self._h, C.byref(vp), C.byref(nv), C.byref(ip), C.byref(nt)))
+        return vp.value, nv.value, ip.value, nt.value
+
+    @property
+    def num_triangles(self) -> int:
+        return self._view()[3]
+
+    @property
+    def num_vertices(self) -> int:
+        return self._view()[1]
+
+    @property
+    def vertices(self) -> np.ndarray:
+        vp, nv, _, _ = self._view()
+        if nv == 0:
+            return np.zeros((0, 3), np.float32)
+        return np.ctypeslib.as_array(C.cast(vp, C.POINTER(C.c_float)), shape=(nv, 3)).copy()
+
+    @property
+    def indices(self) -> np.ndarray:
+        _, _, ip, nt = self._view()
+        if nt == 0:
+            return np.zeros((0, 3), np.int32)
+        return np.ctypeslib.as_array(C.cast(ip, C.POINTER(C.c_int32)), shape=(nt, 3)).copy()
+
+    def bounds(self) -> tuple[np.ndarray, np.ndarray]:
+        lo = np.zeros(3, np.float32)
+        hi = np.zeros(3, np.float32)
+        check(lib.cwbvh_mesh_bounds(self._h, _ptr(lo), _ptr(hi)))
+        return lo, hi
+
+    def diagonal(self) -> float:
+        lo, hi = self.bounds()
+        return float(np.linalg.norm(hi.astype(np.float64) - lo.astype(np.float64)))
+
+
+# -------------------------------------------------------------------- rays
+
+
+def rays_ao_surface(mesh: Mesh, num_points: int, rays_per_point: int, offset: float, tmin: float, tmax: float,
+                    seed: int = 0) -> np.ndarray:
+    out = np.empty((num_points * rays_per_point, 8), np.float32)
+    check(lib.cwbvh_rays_ao_surface(mesh.handle, num_points, rays_per_point, offset, tmin, tmax, seed, _ptr(out)))
+    return out
+
+
+def rays_pinhole(width: int, height: int, eye, target, up, fov_deg: float, tmin=0.0, tmax=1e30) -> np.ndarray:
+    out = np.empty((width * height, 8), np.float32)
+    e = _f32(eye)
+    t = _f32(target)
+    u = _f32(up)
+    check(lib.cwbvh_rays_pinhole(width, height, _ptr(e), _ptr(t), _ptr(u), fov_deg, tmin, tmax, _ptr(out)))
+    return out
+
+
+def rays_ao_from_hits(mesh: Mesh, primary: np.ndarray, hits: np.ndarray, prim_ids: np.ndarray, rays_per_hit: int,
+                      offset: float, tmax: float, seed: int = 0) -> np.ndarray:
+    primary = np.ascontiguousarray(primary, np.float32)
+    hits = np.ascontiguousarray(hits, np.float32)
+    prim_ids = np.ascontiguousarray(prim_ids, np.int32)
+    n = len(primary)
+    out = np.empty((n * rays_per_hit, 8), np.float32)
+    cnt = C.c_int64()
+    check(lib.cwbvh_rays_ao_from_hits(mesh.handle, _ptr(primary), _ptr(hits), _ptr(prim_ids), n, rays_per_hit, offset,
+                                      tmax, seed, _ptr(out), C.byref(cnt)))
+    return out[: cnt.value]
+
+
+def rays_sphere_from_surface(mesh: Mesh, num_rays: int, offset: float, tmin: float = 0.0, tmax: float = np.inf,
+                             seed: int = 0) -> np.ndarray:
+    out = np.empty((num_rays, 8), np.float32)
+    check(lib.cwbvh_rays_sphere_from_surface(mesh.handle, num_rays, offset, tmin, tmax, seed, _ptr(out)))
+    return out
+
+
+def rays_validation(count: int, num_triangles: int) -> np.ndarray:
+    out = np.empty((count, 8), np.float32)
+    check(lib.cwbvh_rays_validation(count, num_triangles, _ptr(out)))
+    return out
+
+
+def shuffle_rays(rays: np.ndarray, seed: int = 0) -> np.ndarray:
+    out = np.ascontiguousarray(rays, np.float32).copy()
+    check(lib.cwbvh_rays_shuffle(_ptr(out), len(out), seed))
+    return out
+
+
+# ------------------------------------------------------------------ layouts
+
+
+def default_build_params(**overrides) -> BuildParams:
+    p = BuildParams()
+    check(lib.cwbvh_build_params_default(C.byref(p)))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+class Layout:
+    """Host GPU layout (reference BVH2 or CWBVH arrays)."""
+
+    def __init__(self, handle: int):
+        self._h = C.c_void_p(handle)
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            lib.cwbvh_layout_free(h)
+            self._h = C.c_void_p(0)
+
+    def view(self) -> LayoutView:
+        v = LayoutView()
+        check(lib.cwbvh_layout_get_view(self._h, C.byref(v)))
+        return v
+
+    @property
+    def kind(self) -> int:
+        return self.view().kind
+
+    @property
+    def node_stride(self) -> int:
+        return self.view().node_stride
+
+    @property
+    def nodes(self) -> np.ndarray:
+        v = self.view()
+        return np.ctypeslib.as_array(C.cast(v.nodes, C.POINTER(C.c_float)), shape=(v.nodes_f4, 4)).copy()
+
+    @property
+    def woop(self) -> np.ndarray:
+        v = self.view()
+        if v.woop_f4 == 0:
+            return np.zeros((0, 4), np.float32)
+        return np.ctypeslib.as_array(C.cast(v.woop, C.POINTER(C.c_float)), shape=(v.woop_f4, 4)).copy()
+
+    @property
+    def prim_idx(self) -> np.ndarray:
+        v = self.view()
+        if v.prim_idx_len == 0:
+            return np.zeros(0, np.int32)
+        return np.ctypeslib.as_array(C.cast(v.prim_idx, C.POINTER(C.c_int32)), shape=(v.prim_idx_len,)).copy()
+
+    def stats(self) -> dict:
+        v = self.view()
+        return {"kind": v.kind, "node_stride": v.node_stride, "nodes_f4": v.nodes_f4, "woop_f4": v.woop_f4,
+                "max_depth": v.max_depth, "stack_need": v.stack_need, "num_nodes": v.num_nodes,
+                "num_leaves": v.num_leaves, "num_tri_refs": v.num_tri_refs, "sah_cost": v.sah_cost}
+
+    def restride(self, node_stride: int) -> "Layout":
+        out = C.c_void_p()
+        check(lib.cwbvh_layout_restride(self._h, node_stride, C.byref(out)))
+        return Layout(out.value)
+
+    @classmethod
+    def from_arrays(cls, kind: int, node_stride: int, nodes, woop, prim_idx) -> "Layout":
+        n = np.ascontiguousarray(nodes, np.float32).reshape(-1, 4)
+        w = np.ascontiguousarray(woop, np.float32).reshape(-1, 4)
+        p = np.ascontiguousarray(prim_idx, np.int32).reshape(-1)
+        out = C.c_void_p()
+        check(lib.cwbvh_layout_from_arrays(kind, node_stride, _ptr(n), len(n), _ptr(w) if len(w) else None, len(w),
+                                           _ptr(p) if len(p) else None, len(p), C.byref(out)))
+        return cls(out.value)
+
+
+def build_layouts(mesh: Mesh, params: BuildParams | None = None, bvh2: bool = True, cwbvh: bool = True):
+    """Build (bvh2_layout, cwbvh_layout) from one SAH tree; either may be None."""
+    a, b = C.c_void_p(), C.c_void_p()
+    check(lib.cwbvh_build_layouts(mesh.handle, C.byref(params) if params is not None else None,
+                                  C.byref(a) if bvh2 else None, C.byref(b) if cwbvh else None))
+    return (Layout(a.value) if bvh2 else None), (Layout(b.value) if cwbvh else None)
+
+
+def woopify(v0, v1, v2) -> np.ndarray:
+    a, b, c = (_f32(x).reshape(3) for x in (v0, v1, v2))
+    out = np.empty((3, 4), np.float32)
+    check(lib.cwbvh_woopify(_ptr(a), _ptr(b), _ptr(c), _ptr(out)))
+    return out
+
+
+# ----------------------------------------------------------------- device
+
+
+def device_count() -> int:
+    n = C.c_int32()
+    check(lib.cwbvh_device_count(C.byref(n)))
+    return n.value
+
+
+def device_info(device: int = 0) -> dict:
+    d = DeviceInfo()
+    check(lib.cwbvh_get_device_info(device, C.byref(d)))
+    return {"name": d.name.decode(), "arch": d.arch.decode(), "compute_units": d.compute_units,
+            "wave_size": d.wave_size, "lds_per_block": d.lds_per_block, "l2_bytes": d.l2_bytes,
+            "global_mem_bytes": d.global_mem_bytes, "clock_khz": d.clock_khz, "mem_clock_khz": d.mem_clock_khz,
+            "mem_bus_width": d.mem_bus_width}
+
+
+class DeviceBuffer:
+    """Owning HIP device allocation."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib.cwbvh_malloc(C.byref(p), self.nbytes))
+        self._p = p
+
+    @property
+    def ptr(self) -> int:
+        return self._p.value or 0
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(max(a.nbytes, 1))
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib.cwbvh_memcpy_htod(self._p, _ptr(a), a.nbytes))
+
+    def download(self, shape, dtype=np.float32) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        assert out.nbytes <= self.nbytes
+        check(lib.cwbvh_memcpy_dtoh(_ptr(out), self._p, out.nbytes))
+        return out
+
+    def zero(self) -> None:
+        check(lib.cwbvh_memset(self._p, 0, self.nbytes))
+
+    def free(self) -> None:
+        if self._p is not None and self._p.value:
+            lib.cwbvh_free(self._p)
+            self._p = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        check(lib.cwbvh_stream_create(C.byref(s)))
+        self._s = s
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._s
+
+    def synchronize(self) -> None:
+        check(lib.cwbvh_stream_synchronize(self._s))
+
+    def __del__(self):
+        s = getattr(self, "_s", None)
+        if s is not None and s.value:
+            lib.cwbvh_stream_destroy(s)
+            self._s = C.c_void_p(0)
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        check(lib.cwbvh_event_create(C.byref(e)))
+        self._e = e
+
+    def record(self, stream: Stream | None = None) -> None:
+        check(lib.cwbvh_event_record(self._e, stream.handle if stream else None))
+
+    def synchronize(self) -> None:
+        check(lib.cwbvh_event_synchronize(self._e))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = C.c_float()
+        check(lib.cwbvh_event_elapsed_ms(self._e, end._e, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        e = getattr(self, "_e", None)
+        if e is not None and e.value:
+            lib.cwbvh_event_destroy(e)
+            self._e = C.c_void_p(0)
+
+
+def synchronize() -> None:
+    check(lib.cwbvh_device_synchronize())
+
+
+def default_trace_config(kind: int, **overrides) -> TraceConfig:
+    c = TraceConfig()
+    check(lib.cwbvh_trace_config_default(kind, C.byref(c)))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+class Scene:
+    """A BVH resident in device memory (owning upload of a host Layout)."""
+
+    def __init__(self, layout: Layout, stream: Stream | None = None):
+        s = C.c_void_p()
+        check(lib.cwbvh_scene_create(layout.handle, stream.handle if stream else None, C.byref(s)))
+        self._s = s
+        self.kind = layout.kind
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._s
+
+    def device_bytes(self) -> int:
+        b = C.c_int64()
+        check(lib.cwbvh_scene_device_bytes(self._s, C.byref(b)))
+        return b.value
+
+    def trace(self, d_rays: int, d_hits: int, n: int, flags: int = TRACE_CLOSEST, cfg: TraceConfig | None = None,
+              d_counters: int | None = None, stream: Stream | None = None, d_order: int | None = None) -> None:
+        cfgp = C.byref(cfg) if cfg is not None else None
+        st = stream.handle if stream else None
+        if d_order is None:
+            check(lib.cwbvh_trace(self._s, d_rays, d_hits, n, flags, cfgp, d_counters, st))
+        else:
+            check(lib.cwbvh_trace_ordered(self._s, d_rays, d_order, d_hits, n, flags, cfgp, d_counters, st))
+
+    def trace_numpy(self, rays: np.ndarray, flags: int = TRACE_CLOSEST, cfg: TraceConfig | None = None,
+                    order: np.ndarray | None = None, with_stats: bool = False):
+        """Upload rays, trace, download hits (n,4) float32; optionally node/tri counts."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        dr = DeviceBuffer.from_numpy(rays)
+        dh = DeviceBuffer(max(16 * n, 16))
+        do = DeviceBuffer.from_numpy(np.ascontiguousarray(order, np.int32)) if order is not None else None
+        dc = None
+        if with_stats:
+            dc = DeviceBuffer(16)
+            dc.zero()
+            flags |= TRACE_STATS
+        self.trace(dr.ptr, dh.ptr, n, flags, cfg, dc.ptr if dc else None, None, do.ptr if do else None)
+        synchronize()
+        hits = dh.download((n, 4), np.float32)
+        if with_stats:
+            return hits, dc.download((2,), np.uint64)
+        return hits
+
+    def __del__(self):
+        s = getattr(self, "_s", None)
+        if s is not None and s.value:
+            lib.cwbvh_scene_destroy(s)
+            self._s = C.c_void_p(0)
+
+
+def stack_overflowed(stream: Stream | None = None) -> bool:
+    f = C.c_int32()
+    check(lib.cwbvh_stack_overflowed(stream.handle if stream else None, C.byref(f)))
+    return bool(f.value)
+
+
+# ------------------------------------------------ reference entry points
+
+
+def rtBindCWBVHData(nodes_ptr: int, woop_ptr: int, map_ptr: int | None) -> None:
+    lib.rtBindCWBVHData(nodes_ptr, woop_ptr, map_ptr)
+    check(lib.cwbvh_last_status())
+
+
+def rtTraceCWBVH(rays_ptr: int, hits_ptr: int, count: int) -> float:
+    lib.rtTraceCWBVH(rays_ptr, hits_ptr, count)
+    check(lib.cwbvh_last_status())
+    return lib.cwbvh_last_trace_ms()
+
+
+def rtBindBVH2Data(nodes_ptr: int, woop_ptr: int, map_ptr: int | None) -> None:
+    lib.rtBindBVH2Data(nodes_ptr, woop_ptr, map_ptr)
+    check(lib.cwbvh_last_status())
+
+
+def rtTraceBVH2(rays_ptr: int, hits_ptr: int, count: int) -> float:
+    lib.rtTraceBVH2(rays_ptr, hits_ptr, count)
+    check(lib.cwbvh_last_status())
+    return lib.cwbvh_last_trace_ms()
+
+
+def set_log(enabled: bool) -> None:
+    lib.cwbvh_set_log(1 if enabled else 0)
+
+
+def hit_ids(hits: np.ndarray) -> np.ndarray:
+    """triAddr (or prim id) column of a hit array as int32."""
+    return np.ascontiguousarray(hits, np.float32).view(np.int32).reshape(-1, 4)[:, 1].copy()

@@ -1,0 +1,390 @@
+// C ABI: errors, device plumbing, device scenes and the trace entry points.
+#include <hip/hip_runtime_api.h>
+
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <thread>
+
+#include "internal.h"
+
+namespace cw {
+
+namespace {
+thread_local std::string g_error;
+}
+
+void set_error(const std::string& msg) { g_error = msg; }
+const char* get_error() { return g_error.c_str(); }
+
+int32_t hip_fail(hipError_t e, const char* what) {
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? CWBVH_ERR_OUT_OF_MEMORY
+                                    : (e == hipErrorNoDevice ? CWBVH_ERR_NO_DEVICE : CWBVH_ERR_HIP);
+}
+
+int num_threads_or_default(int requested) {
+    if (requested > 0) return requested;
+    unsigned h = std::thread::hardware_concurrency();
+    return h ? (int)h : 1;
+}
+
+int device_compute_units() {
+    static int cus = 0;
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            cus = prop.multiProcessorCount;
+        if (cus <= 0) cus = 1;
+    });
+    return cus;
+}
+
+int32_t stream_counters(hipStream_t stream, unsigned** out) {
+    static std::mutex mu;
+    static std::map<hipStream_t, unsigned*> table;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = table.find(stream);
+    if (it != table.end()) {
+        *out = it->second;
+        return CWBVH_OK;
+    }
+    unsigned* p = nullptr;
+    CW_HIP(hipMalloc(&p, 4 * sizeof(unsigned)));
+    CW_HIP(hipMemset(p, 0, 4 * sizeof(unsigned)));
+    table[stream] = p;
+    *out = p;
+    return CWBVH_OK;
+}
+
+}  // namespace cw
+
+using namespace cw;
+
+extern "C" {
+
+int32_t cwbvh_abi_version(void) { return CWBVH_ABI_VERSION; }
+
+const char* cwbvh_status_string(int32_t s) {
+    switch (s) {
+        case CWBVH_OK: return "ok";
+        case CWBVH_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case CWBVH_ERR_HIP: return "HIP runtime error";
+        case CWBVH_ERR_OUT_OF_MEMORY: return "out of device memory";
+        case CWBVH_ERR_STACK_DEPTH: return "tree deeper than the traversal stack";
+        case CWBVH_ERR_NOT_BOUND: return "no BVH bound";
+        case CWBVH_ERR_NO_DEVICE: return "no HIP device";
+        case CWBVH_ERR_BUDGET: return "request exceeds the memory budget";
+        case CWBVH_ERR_CANCELLED: return "cancelled";
+        case CWBVH_ERR_INTERNAL: return "internal error";
+        default: return "unknown status";
+    }
+}
+
+const char* cwbvh_last_error(void) { return cw::get_error(); }
+
+// ------------------------------------------------------------ plumbing
+
+int32_t cwbvh_device_count(int32_t* count) {
+    CW_CHECK_ARG(count != nullptr, "count is NULL");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e == hipErrorNoDevice || e == hipErrorInvalidDevice) n = 0;
+    else if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    *count = n;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_get_device_info(int32_t device, cwbvh_device_info* info) {
+    CW_CHECK_ARG(info != nullptr, "info is NULL");
+    hipDeviceProp_t p;
+    CW_HIP(hipGetDeviceProperties(&p, device));
+    std::memset(info, 0, sizeof(*info));
+    std::snprintf(info->name, sizeof(info->name), "%s", p.name);
+    std::snprintf(info->arch, sizeof(info->arch), "%s", p.gcnArchName);
+    info->compute_units = p.multiProcessorCount;
+    info->wave_size = p.warpSize;
+    info->lds_per_block = (int64_t)p.sharedMemPerBlock;
+    info->l2_bytes = (int64_t)p.l2CacheSize;
+    info->global_mem_bytes = (int64_t)p.totalGlobalMem;
+    info->clock_khz = p.clockRate;
+    info->mem_clock_khz = p.memoryClockRate;
+    info->mem_bus_width = p.memoryBusWidth;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_malloc(void** dptr, int64_t bytes) {
+    CW_CHECK_ARG(dptr != nullptr && bytes >= 0, "bad malloc arguments");
+    *dptr = nullptr;
+    if (bytes == 0) return CWBVH_OK;
+    CW_HIP(hipMalloc(dptr, (size_t)bytes));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_free(void* dptr) {
+    if (dptr) CW_HIP(hipFree(dptr));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_memcpy_htod(void* dst, const void* src, int64_t bytes) {
+    CW_CHECK_ARG(bytes >= 0 && (bytes == 0 || (dst && src)), "bad copy arguments");
+    if (bytes) CW_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_memcpy_dtoh(void* dst, const void* src, int64_t bytes) {
+    CW_CHECK_ARG(bytes >= 0 && (bytes == 0 || (dst && src)), "bad copy arguments");
+    if (bytes) CW_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_memcpy_dtod(void* dst, const void* src, int64_t bytes) {
+    CW_CHECK_ARG(bytes >= 0 && (bytes == 0 || (dst && src)), "bad copy arguments");
+    if (bytes) CW_HIP(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToDevice));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_memset(void* dptr, int32_t value, int64_t bytes) {
+    CW_CHECK_ARG(bytes >= 0 && (bytes == 0 || dptr), "bad memset arguments");
+    if (bytes) CW_HIP(hipMemset(dptr, value, (size_t)bytes));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_stream_create(void** stream) {
+    CW_CHECK_ARG(stream != nullptr, "stream is NULL");
+    hipStream_t s;
+    CW_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_stream_destroy(void* stream) {
+    if (stream) CW_HIP(hipStreamDestroy((hipStream_t)stream));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_stream_synchronize(void* stream) {
+    CW_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_device_synchronize(void) {
+    CW_HIP(hipDeviceSynchronize());
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_event_create(void** event) {
+    CW_CHECK_ARG(event != nullptr, "event is NULL");
+    hipEvent_t e;
+    CW_HIP(hipEventCreate(&e));
+    *event = e;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_event_destroy(void* event) {
+    if (event) CW_HIP(hipEventDestroy((hipEvent_t)event));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_event_record(void* event, void* stream) {
+    CW_CHECK_ARG(event != nullptr, "event is NULL");
+    CW_HIP(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_event_synchronize(void* event) {
+    CW_CHECK_ARG(event != nullptr, "event is NULL");
+    CW_HIP(hipEventSynchronize((hipEvent_t)event));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_event_elapsed_ms(void* start, void* stop, float* ms) {
+    CW_CHECK_ARG(start && stop && ms, "NULL argument");
+    CW_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return CWBVH_OK;
+}
+
+// ------------------------------------------------------------ scenes
+
+int32_t cwbvh_scene_create(const cwbvh_layout* L, void* stream, cwbvh_scene** out) {
+    CW_CHECK_ARG(L != nullptr && out != nullptr, "NULL argument");
+    CW_CHECK_ARG(!L->nodes.empty(), "empty node array");
+    const int cap = L->kind == CWBVH_KIND_CWBVH ? CWBVH_STACK_CAPACITY : CWBVH_BVH2_STACK_CAPACITY;
+    if (L->stack_need > cap) {
+        set_error("tree needs " + std::to_string(L->stack_need) + " stack entries, kernel holds " +
+                  std::to_string(cap));
+        return CWBVH_ERR_STACK_DEPTH;
+    }
+    auto* s = new cwbvh_scene();
+    s->kind = L->kind;
+    s->node_stride = L->node_stride;
+    s->stack_need = L->stack_need;
+    s->owning = true;
+    s->woop_f4 = (int64_t)L->woop.size();
+    hipGetDevice(&s->device);
+    const size_t nb = L->nodes.size() * sizeof(cwbvh_float4);
+    // two float4 of zero padding after the Woop rows keep every triangle-sized
+    // read of the last leaf inside the allocation
+    const size_t wb = (L->woop.size() + 2) * sizeof(cwbvh_float4);
+    const size_t pb = std::max<size_t>(1, L->prim_idx.size()) * sizeof(int32_t);
+    void *dn = nullptr, *dw = nullptr, *dp = nullptr;
+    hipError_t e = hipMalloc(&dn, nb);
+    if (e == hipSuccess) e = hipMalloc(&dw, wb);
+    if (e == hipSuccess) e = hipMalloc(&dp, pb);
+    hipStream_t st = (hipStream_t)stream;
+    if (e == hipSuccess) e = hipMemcpyAsync(dn, L->nodes.data(), nb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemsetAsync(dw, 0, wb, st);
+    if (e == hipSuccess && !L->woop.empty())
+        e = hipMemcpyAsync(dw, L->woop.data(), L->woop.size() * sizeof(cwbvh_float4), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !L->prim_idx.empty())
+        e = hipMemcpyAsync(dp, L->prim_idx.data(), L->prim_idx.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+        if (dn) hipFree(dn);
+        if (dw) hipFree(dw);
+        if (dp) hipFree(dp);
+        delete s;
+        return hip_fail(e, "scene upload");
+    }
+    s->nodes = dn;
+    s->woop = dw;
+    s->prim_idx = (const int32_t*)dp;
+    s->bytes = (int64_t)(nb + wb + pb);
+    *out = s;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_scene_wrap_device(int32_t kind, int32_t node_stride, const void* d_nodes, const void* d_woop,
+                                const int32_t* d_prim_idx, int32_t stack_need, cwbvh_scene** out) {
+    CW_CHECK_ARG(out != nullptr && d_nodes != nullptr && d_woop != nullptr, "NULL argument");
+    CW_CHECK_ARG(kind == CWBVH_KIND_BVH2 ? node_stride == 4
+                                         : (kind == CWBVH_KIND_CWBVH && (node_stride == 5 || node_stride == 8)),
+                 "bad kind / node stride");
+    const int cap = kind == CWBVH_KIND_CWBVH ? CWBVH_STACK_CAPACITY : CWBVH_BVH2_STACK_CAPACITY;
+    if (stack_need > cap) {
+        set_error("tree deeper than the traversal stack");
+        return CWBVH_ERR_STACK_DEPTH;
+    }
+    auto* s = new cwbvh_scene();
+    s->kind = kind;
+    s->node_stride = node_stride;
+    s->nodes = d_nodes;
+    s->woop = d_woop;
+    s->prim_idx = d_prim_idx;
+    s->stack_need = stack_need;
+    s->owning = false;
+    hipGetDevice(&s->device);
+    // valid extent of the Woop array: up to the end of its allocation
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)d_woop) == hipSuccess && size > 0) {
+        size_t off = (size_t)((const char*)d_woop - (const char*)base);
+        s->woop_f4 = (int64_t)((size - off) / sizeof(cwbvh_float4));
+    }
+    hipGetLastError();
+    *out = s;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_scene_device_bytes(const cwbvh_scene* s, int64_t* bytes) {
+    CW_CHECK_ARG(s != nullptr && bytes != nullptr, "NULL argument");
+    *bytes = s->bytes;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_scene_kind(const cwbvh_scene* s, int32_t* kind) {
+    CW_CHECK_ARG(s != nullptr && kind != nullptr, "NULL argument");
+    *kind = s->kind;
+    return CWBVH_OK;
+}
+
+void cwbvh_scene_destroy(cwbvh_scene* s) {
+    if (!s) return;
+    if (s->owning) {
+        hipFree(const_cast<void*>(s->nodes));
+        hipFree(const_cast<void*>(s->woop));
+        hipFree(const_cast<int32_t*>(s->prim_idx));
+    }
+    delete s;
+}
+
+// ------------------------------------------------------------- trace
+
+int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* c) {
+    CW_CHECK_ARG(c != nullptr, "cfg is NULL");
+    CW_CHECK_ARG(kind == CWBVH_KIND_BVH2 || kind == CWBVH_KIND_CWBVH, "bad kind");
+    std::memset(c, 0, sizeof(*c));
+    if (kind == CWBVH_KIND_CWBVH) {
+        c->block_size = 128;
+        c->lds_depth = 16;
+        c->grid_blocks = 0;
+        c->fetch_chunk = 64;
+        c->refill_idle = 8;
+        c->refill_budget = 32;
+        c->postpone_pct = 0;
+        c->prefetch = 0;
+    } else {
+        c->block_size = 128;
+        c->lds_depth = 16;
+    }
+    return CWBVH_OK;
+}
+
+static int32_t trace_impl(const cwbvh_scene* scene, const cwbvh_ray* d_rays, const int32_t* d_order,
+                          cwbvh_hit* d_hits, int64_t n, uint32_t flags, const cwbvh_trace_config* cfg,
+                          uint64_t* d_counters, void* stream) {
+    CW_CHECK_ARG(scene != nullptr, "scene is NULL");
+    CW_CHECK_ARG(n >= 0 && n <= 0x7FFFFFFF, "ray count must be in [0, 2^31)");
+    CW_CHECK_ARG(n == 0 || (d_rays != nullptr && d_hits != nullptr), "NULL ray or hit buffer");
+    CW_CHECK_ARG((flags & ~(uint32_t)(CWBVH_TRACE_ANY | CWBVH_TRACE_PRIM_ID | CWBVH_TRACE_STATS)) == 0, "unknown flags");
+    CW_CHECK_ARG(!(flags & CWBVH_TRACE_STATS) || d_counters != nullptr, "STATS needs a counter buffer");
+    CW_CHECK_ARG(!(flags & CWBVH_TRACE_PRIM_ID) || scene->prim_idx != nullptr, "PRIM_ID needs a prim index array");
+    cwbvh_trace_config c;
+    if (cfg) c = *cfg;
+    else cwbvh_trace_config_default(scene->kind, &c);
+    int32_t rc = validate_config(scene->kind, c);
+    if (rc) return rc;
+    if (n == 0) return CWBVH_OK;
+    TraceLaunch L{};
+    L.scene = scene;
+    L.rays = d_rays;
+    L.order = d_order;
+    L.hits = d_hits;
+    L.n = n;
+    L.flags = flags;
+    L.cfg = c;
+    L.stats = d_counters;
+    L.stream = (hipStream_t)stream;
+    rc = stream_counters(L.stream, &L.counters);
+    if (rc) return rc;
+    return scene->kind == CWBVH_KIND_CWBVH ? launch_cwbvh(L) : launch_bvh2(L);
+}
+
+int32_t cwbvh_trace(const cwbvh_scene* scene, const cwbvh_ray* d_rays, cwbvh_hit* d_hits, int64_t n, uint32_t flags,
+                    const cwbvh_trace_config* cfg, uint64_t* d_counters, void* stream) {
+    return trace_impl(scene, d_rays, nullptr, d_hits, n, flags, cfg, d_counters, stream);
+}
+
+int32_t cwbvh_trace_ordered(const cwbvh_scene* scene, const cwbvh_ray* d_rays, const int32_t* d_order,
+                            cwbvh_hit* d_hits, int64_t n, uint32_t flags, const cwbvh_trace_config* cfg,
+                            uint64_t* d_counters, void* stream) {
+    CW_CHECK_ARG(d_order != nullptr || n == 0, "order is NULL");
+    return trace_impl(scene, d_rays, d_order, d_hits, n, flags, cfg, d_counters, stream);
+}
+
+int32_t cwbvh_stack_overflowed(void* stream, int32_t* flag) {
+    CW_CHECK_ARG(flag != nullptr, "flag is NULL");
+    unsigned* c = nullptr;
+    int32_t rc = stream_counters((hipStream_t)stream, &c);
+    if (rc) return rc;
+    unsigned v = 0;
+    CW_HIP(hipMemcpyAsync(&v, c + 2, sizeof(unsigned), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    CW_HIP(hipStreamSynchronize((hipStream_t)stream));
+    *flag = v ? 1 : 0;
+    if (v) CW_HIP(hipMemsetAsync(c + 2, 0, sizeof(unsigned), (hipStream_t)stream));
+    return CWBVH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,796 @@
+// SAH BVH builder and GPU layout converters (host).
+//
+// Replaces the reference's Embree-based builder (src/EmbreeBVHBuilder.cpp:303-478,
+// which needs Embree3 + TBB) and its converters (src/GPUBVHConverter.cpp:10-403).
+//
+// One binned-SAH binary tree with single-triangle leaves is built per mesh.
+// Two products are derived from it:
+//   * BVH2: leaves merged bottom-up wherever the SAH leaf cost is lower
+//     (bounded by bvh2_max_leaf), written in the reference BVH2 layout
+//     (GPUBVHConverter.cpp:10-121): 4 float4 per node, child ints >= 0 are
+//     float4 node addresses, < 0 are ~(Woop float4 index); leaves are Woop
+//     triangles closed by a 0x80000000 terminator float4.
+//   * CWBVH: the binary tree collapsed to 8-wide nodes with <= 3 triangles per
+//     leaf by the dynamic program of Ylitie, Karras, Laine (HPG 2017, Sec. 3.1),
+//     written in the reference compressed layout (GPUBVHConverter.cpp:123-378):
+//       n0 = (p.xyz, e.x e.y e.z imask bytes), n1 = (child base node index,
+//       triangle base index, meta[0..3], meta[4..7]), n2..n4 = 8-bit child
+//       bounds qlo.x[8] qlo.y[8] qlo.z[8] qhi.x[8] qhi.y[8] qhi.z[8].
+//     Inner-child meta = 0b001_11sss (slot s), leaf meta = unary(count)<<5 |
+//     triangle offset.  Children are assigned to octant slots greedily by
+//     centroid direction.  Optionally each node is padded to 8 float4 (128 B).
+//
+// Deviations from the reference converter, all keeping the layout:
+//   * per-axis exponent is the smallest e with extent <= 255*2^e, clamped to
+//     [-126, 127] (reference: ceilf(log2f(extent/255)) gives -inf -> UB for a
+//     flat axis and can round to a qhi of 256, stored as 0);
+//   * quantization runs in double so floor/ceil are conservative;
+//   * a root that is a leaf, and an empty mesh, get a valid inner root;
+//   * no fixed-size DFS stacks;
+//   * Woop row0.x of -0.0 is stored as +0.0 so it cannot read as a terminator.
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <mutex>
+#include <numeric>
+
+#include "common.h"
+
+using namespace cw;
+
+namespace {
+
+struct BNode {
+    Box box;
+    int32_t left = -1, right = -1;  // internal: children
+    int32_t first = 0, count = 0;   // leaf: range in refs
+    int32_t prims = 0;              // triangles in subtree
+};
+
+struct Tree {
+    std::vector<BNode> nodes;
+    std::vector<int32_t> refs;
+    int32_t root = 0;
+};
+
+struct Builder {
+    const std::vector<Box>& pbox;
+    const std::vector<V3>& pcen;
+    std::vector<BNode>& nodes;
+    std::vector<int32_t>& refs;
+    std::atomic<int32_t> next_node{0};
+    std::atomic<int> live_threads{1};
+    int max_threads;
+    int bins;
+    int max_depth;
+
+    Builder(const std::vector<Box>& b, const std::vector<V3>& c, std::vector<BNode>& n, std::vector<int32_t>& r,
+            int threads, int nbins, int maxd)
+        : pbox(b), pcen(c), nodes(n), refs(r), max_threads(threads), bins(nbins), max_depth(maxd) {}
+
+    int32_t alloc2() { return next_node.fetch_add(2); }
+
+    struct Bin {
+        Box box;
+        int32_t n = 0;
+    };
+
+    // Binned SAH over [b, e).  Returns the partition point or -1 for "no good
+    // split" (then the caller splits at the median).
+    int32_t split(int32_t b, int32_t e, const Box& cb, int depth) {
+        int32_t n = e - b;
+        int log2n = 0;
+        while ((1 << log2n) < n) ++log2n;
+        bool force_median = depth + log2n >= max_depth;
+        int best_axis = -1, best_bin = -1;
+        float best_cost = INFINITY;
+        if (!force_median) {
+            int nb = std::min(bins, std::max(4, n));
+            std::vector<Bin> bin(3 * nb);
+            auto bin_range = [&](int32_t lo, int32_t hi, std::vector<Bin>& out) {
+                for (int32_t i = lo; i < hi; ++i) {
+                    int32_t p = refs[i];
+                    for (int a = 0; a < 3; ++a) {
+                        float lo_a = comp(cb.lo, a), ext = comp(cb.hi, a) - lo_a;
+                        if (!(ext > 0)) continue;
+                        int k = (int)((comp(pcen[p], a) - lo_a) * ((float)nb / ext));
+                        k = std::min(nb - 1, std::max(0, k));
+                        Bin& bb = out[a * nb + k];
+                        bb.box.grow(pbox[p]);
+                        bb.n++;
+                    }
+                }
+            };
+            if (n >= (1 << 18) && max_threads > 1) {
+                int t = max_threads;
+                std::vector<std::vector<Bin>> part(t, std::vector<Bin>(3 * nb));
+                std::vector<std::thread> pool;
+                int32_t chunk = (n + t - 1) / t;
+                for (int i = 0; i < t; ++i) {
+                    int32_t lo = b + i * chunk, hi = std::min(e, lo + chunk);
+                    if (lo >= hi) break;
+                    pool.emplace_back([&, lo, hi, i] { bin_range(lo, hi, part[i]); });
+                }
+                for (auto& th : pool) th.join();
+                for (auto& pt : part)
+                    for (int k = 0; k < 3 * nb; ++k) {
+                        bin[k].box.grow(pt[k].box);
+                        bin[k].n += pt[k].n;
+                    }
+            } else {
+                bin_range(b, e, bin);
+            }
+            std::vector<float> right_cost(nb);
+            for (int a = 0; a < 3; ++a) {
+                if (!(comp(cb.hi, a) - comp(cb.lo, a) > 0)) continue;
+                Box acc;
+                int32_t cnt = 0;
+                for (int k = nb - 1; k >= 1; --k) {
+                    acc.grow(bin[a * nb + k].box);
+                    cnt += bin[a * nb + k].n;
+                    right_cost[k] = cnt ? acc.area() * (float)cnt : 0.f;
+                }
+                acc = Box();
+                cnt = 0;
+                for (int k = 0; k < nb - 1; ++k) {
+                    acc.grow(bin[a * nb + k].box);
+                    cnt += bin[a * nb + k].n;
+                    if (cnt == 0 || cnt == n) continue;
+                    float c = acc.area() * (float)cnt + right_cost[k + 1];
+                    if (c < best_cost) {
+                        best_cost = c;
+                        best_axis = a;
+                        best_bin = k;
+                    }
+                }
+            }
+        }
+        if (best_axis < 0) return -1;
+        int nb = std::min(bins, std::max(4, n));
+        float lo_a = comp(cb.lo, best_axis), ext = comp(cb.hi, best_axis) - lo_a;
+        auto mid = std::partition(refs.begin() + b, refs.begin() + e, [&](int32_t p) {
+            int k = (int)((comp(pcen[p], best_axis) - lo_a) * ((float)nb / ext));
+            k = std::min(nb - 1, std::max(0, k));
+            return k <= best_bin;
+        });
+        int32_t m = (int32_t)(mid - refs.begin());
+        if (m == b || m == e) return -1;
+        return m;
+    }
+
+    void build(int32_t idx, int32_t b, int32_t e, int depth) {
+        BNode& nd = nodes[idx];
+        Box box, cb;
+        for (int32_t i = b; i < e; ++i) {
+            box.grow(pbox[refs[i]]);
+            cb.grow(pcen[refs[i]]);
+        }
+        nd.box = box;
+        nd.prims = e - b;
+        if (e - b == 1) {
+            nd.first = b;
+            nd.count = 1;
+            return;
+        }
+        int32_t m = split(b, e, cb, depth);
+        if (m < 0) {
+            // median split along the widest centroid axis
+            V3 d = cb.hi - cb.lo;
+            int a = (d.x >= d.y && d.x >= d.z) ? 0 : (d.y >= d.z ? 1 : 2);
+            m = b + (e - b) / 2;
+            std::nth_element(refs.begin() + b, refs.begin() + m, refs.begin() + e, [&](int32_t p, int32_t q) {
+                float cp = comp(pcen[p], a), cq = comp(pcen[q], a);
+                return cp < cq || (cp == cq && p < q);
+            });
+        }
+        int32_t c = alloc2();
+        nd.left = c;
+        nd.right = c + 1;
+        bool spawn = (e - b) > 32768 && live_threads.load() < max_threads;
+        if (spawn) {
+            live_threads.fetch_add(1);
+            std::thread t([this, c, b, m, depth] { build(c, b, m, depth + 1); });
+            build(c + 1, m, e, depth + 1);
+            t.join();
+            live_threads.fetch_sub(1);
+        } else {
+            build(c, b, m, depth + 1);
+            build(c + 1, m, e, depth + 1);
+        }
+    }
+};
+
+void build_tree(const cwbvh_mesh* mesh, const cwbvh_build_params& P, Tree& T) {
+    int64_t nt = mesh->nt();
+    std::vector<Box> pbox(nt);
+    std::vector<V3> pcen(nt);
+    int threads = num_threads_or_default(P.num_threads);
+    parallel_for(nt, threads, [&](int64_t b, int64_t e) {
+        for (int64_t t = b; t < e; ++t) {
+            V3 a, bb, c;
+            mesh->tri(t, a, bb, c);
+            Box x;
+            x.grow(a);
+            x.grow(bb);
+            x.grow(c);
+            pbox[t] = x;
+            pcen[t] = x.center();
+        }
+    });
+    T.refs.resize(nt);
+    std::iota(T.refs.begin(), T.refs.end(), 0);
+    T.nodes.assign(std::max<int64_t>(1, 2 * nt - 1), BNode());
+    T.root = 0;
+    if (nt == 0) {
+        T.nodes.resize(1);
+        T.nodes[0].count = 0;
+        T.nodes[0].prims = 0;
+        return;
+    }
+    Builder B(pbox, pcen, T.nodes, T.refs, threads, P.sah_bins, P.max_depth);
+    B.next_node = 1;
+    B.build(0, 0, (int32_t)nt, 1);
+    T.nodes.resize(B.next_node.load());
+}
+
+// ------------------------------------------------------------------ Woop
+
+void woopify(V3 v0, V3 v1, V3 v2, cwbvh_float4 out[3]) {
+    // Rows of A^-1 for A = [v0-v2 | v1-v2 | n], n = (v0-v2)x(v1-v2):
+    //   r0 = (e1 x n)/|n|^2, r1 = (n x e0)/|n|^2, r2 = n/|n|^2.
+    // Local coordinates (u, v, w) of x are (r0, r1, r2) . (x - v2).
+    double e0[3] = {(double)v0.x - v2.x, (double)v0.y - v2.y, (double)v0.z - v2.z};
+    double e1[3] = {(double)v1.x - v2.x, (double)v1.y - v2.y, (double)v1.z - v2.z};
+    double n[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};
+    double det = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+    if (!(det > 0.0) || !std::isfinite(det)) {
+        // degenerate: Dz = 0 for every ray, so t is never finite inside (tmin, tmax)
+        out[0] = cwbvh_float4{0.f, 0.f, 0.f, 1.f};
+        out[1] = cwbvh_float4{0.f, 0.f, 0.f, 0.f};
+        out[2] = cwbvh_float4{0.f, 0.f, 0.f, 0.f};
+        return;
+    }
+    double r0[3] = {(e1[1] * n[2] - e1[2] * n[1]) / det, (e1[2] * n[0] - e1[0] * n[2]) / det,
+                    (e1[0] * n[1] - e1[1] * n[0]) / det};
+    double r1[3] = {(n[1] * e0[2] - n[2] * e0[1]) / det, (n[2] * e0[0] - n[0] * e0[2]) / det,
+                    (n[0] * e0[1] - n[1] * e0[0]) / det};
+    double r2[3] = {n[0] / det, n[1] / det, n[2] / det};
+    double p[3] = {v2.x, v2.y, v2.z};
+    auto d3 = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+    out[0] = cwbvh_float4{(float)r2[0], (float)r2[1], (float)r2[2], (float)d3(r2, p)};
+    out[1] = cwbvh_float4{(float)r0[0], (float)r0[1], (float)r0[2], (float)-d3(r0, p)};
+    out[2] = cwbvh_float4{(float)r1[0], (float)r1[1], (float)r1[2], (float)-d3(r1, p)};
+    if (f2u(out[0].x) == 0x80000000u) out[0].x = 0.0f;
+}
+
+void append_tri(const cwbvh_mesh* mesh, int32_t prim, cwbvh_layout& L) {
+    V3 a, b, c;
+    mesh->tri(prim, a, b, c);
+    cwbvh_float4 w[3];
+    woopify(a, b, c, w);
+    L.woop.push_back(w[0]);
+    L.woop.push_back(w[1]);
+    L.woop.push_back(w[2]);
+    L.prim_idx.push_back(prim);
+    L.prim_idx.push_back(-1);
+    L.prim_idx.push_back(-1);
+    L.num_tri_refs++;
+}
+
+// ------------------------------------------------------------------ BVH2
+
+void layout_bvh2(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params& P, cwbvh_layout& L) {
+    const auto& N = T.nodes;
+    int32_t nn = (int32_t)N.size();
+    // bottom-up SAH: make a leaf wherever it is cheaper (children indices > parent)
+    std::vector<double> cost(nn);
+    std::vector<uint8_t> leaf(nn, 0);
+    for (int32_t i = nn - 1; i >= 0; --i) {
+        const BNode& nd = N[i];
+        double A = nd.box.area();
+        double lc = A * nd.prims * P.bvh2_tri_cost;
+        if (nd.left < 0) {
+            cost[i] = lc;
+            leaf[i] = 1;
+            continue;
+        }
+        double ic = A * P.bvh2_node_cost + cost[nd.left] + cost[nd.right];
+        if (nd.prims <= P.bvh2_max_leaf && lc <= ic) {
+            cost[i] = lc;
+            leaf[i] = 1;
+        } else {
+            cost[i] = ic;
+        }
+    }
+    double root_area = N[T.root].box.area();
+    L.sah_cost = root_area > 0 ? cost[T.root] / root_area : 0.0;
+
+    L.kind = CWBVH_KIND_BVH2;
+    L.node_stride = 4;
+    auto gather = [&](int32_t n, std::vector<int32_t>& out) {
+        std::vector<int32_t> st{n};
+        while (!st.empty()) {
+            int32_t x = st.back();
+            st.pop_back();
+            if (N[x].left < 0) {
+                for (int32_t k = 0; k < N[x].count; ++k) out.push_back(T.refs[N[x].first + k]);
+            } else {
+                st.push_back(N[x].right);
+                st.push_back(N[x].left);
+            }
+        }
+    };
+    auto emit_leaf = [&](const std::vector<int32_t>& prims) -> int32_t {
+        int32_t addr = (int32_t)L.woop.size();
+        for (int32_t p : prims) append_tri(mesh, p, L);
+        L.woop.push_back(cwbvh_float4{u2f(0x80000000u), 0.f, 0.f, 0.f});
+        L.prim_idx.push_back(-1);
+        L.num_leaves++;
+        return ~addr;
+    };
+    auto put_child = [&](int32_t node_addr, int slot, int32_t value) {
+        if (slot == 0) L.nodes[node_addr + 3].x = i2f(value);
+        else L.nodes[node_addr + 3].y = i2f(value);
+    };
+    auto put_box = [&](int32_t node_addr, int slot, const Box& b) {
+        if (slot == 0) {
+            L.nodes[node_addr + 0] = cwbvh_float4{b.lo.x, b.hi.x, b.lo.y, b.hi.y};
+            L.nodes[node_addr + 2].x = b.lo.z;
+            L.nodes[node_addr + 2].y = b.hi.z;
+        } else {
+            L.nodes[node_addr + 1] = cwbvh_float4{b.lo.x, b.hi.x, b.lo.y, b.hi.y};
+            L.nodes[node_addr + 2].z = b.lo.z;
+            L.nodes[node_addr + 2].w = b.hi.z;
+        }
+    };
+    auto new_node = [&]() {
+        int32_t a = (int32_t)L.nodes.size();
+        for (int k = 0; k < 4; ++k) L.nodes.push_back(cwbvh_float4{0.f, 0.f, 0.f, 0.f});
+        L.num_nodes++;
+        return a;
+    };
+    Box inverted;
+    inverted.lo = V3{FLT_MAX, FLT_MAX, FLT_MAX};
+    inverted.hi = V3{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+
+    const BNode& root = N[T.root];
+    if (root.prims == 0 || leaf[T.root]) {
+        // wrap: inner root whose second child can never be hit
+        int32_t r = new_node();
+        std::vector<int32_t> prims;
+        if (root.prims) gather(T.root, prims);
+        put_box(r, 0, root.prims ? root.box : inverted);
+        put_child(r, 0, emit_leaf(prims));
+        put_box(r, 1, inverted);
+        put_child(r, 1, emit_leaf({}));
+        L.max_depth = 1;
+        L.stack_need = 2;
+        return;
+    }
+    // pre-order DFS; stack holds (tree node, parent address, slot, depth)
+    struct Item { int32_t n, parent, slot, depth; };
+    std::vector<Item> st{{T.root, -1, 0, 1}};
+    int32_t maxd = 0;
+    while (!st.empty()) {
+        Item it = st.back();
+        st.pop_back();
+        const BNode& nd = N[it.n];
+        if (leaf[it.n]) {
+            std::vector<int32_t> prims;
+            gather(it.n, prims);
+            put_child(it.parent, it.slot, emit_leaf(prims));
+            continue;
+        }
+        int32_t a = new_node();
+        maxd = std::max(maxd, it.depth);
+        if (it.parent >= 0) put_child(it.parent, it.slot, a);
+        put_box(a, 0, N[nd.left].box);
+        put_box(a, 1, N[nd.right].box);
+        st.push_back({nd.right, a, 1, it.depth + 1});
+        st.push_back({nd.left, a, 0, it.depth + 1});
+    }
+    L.max_depth = maxd;
+    L.stack_need = maxd + 1;
+}
+
+// ----------------------------------------------------------------- CWBVH
+
+struct Collapse {
+    const Tree& T;
+    const cwbvh_build_params& P;
+    std::vector<float> C;        // C[n*8 + i], i = 1..7 (index 0 unused)
+    std::vector<uint8_t> dec;    // dec[n*8 + i]: i=1: 1 = leaf, 0 = inner; i>=2: split k (0 = use i-1)
+    std::vector<uint8_t> split8;
+
+    Collapse(const Tree& t, const cwbvh_build_params& p) : T(t), P(p) {}
+
+    void run() {
+        const auto& N = T.nodes;
+        int32_t nn = (int32_t)N.size();
+        C.assign((size_t)nn * 8, 0.f);
+        dec.assign((size_t)nn * 8, 0);
+        split8.assign(nn, 0);
+        for (int32_t n = nn - 1; n >= 0; --n) {
+            const BNode& nd = N[n];
+            float A = nd.box.area();
+            float* c = &C[(size_t)n * 8];
+            uint8_t* d = &dec[(size_t)n * 8];
+            if (nd.left < 0) {
+                float lc = A * nd.prims * P.cwbvh_tri_cost;
+                for (int i = 1; i < 8; ++i) c[i] = lc;
+                d[1] = 1;
+                continue;
+            }
+            const float* cl = &C[(size_t)nd.left * 8];
+            const float* cr = &C[(size_t)nd.right * 8];
+            float dist[9];
+            uint8_t dk[9];
+            for (int j = 2; j <= 8; ++j) {
+                float best = INFINITY;
+                uint8_t bk = 1;
+                for (int k = 1; k < j; ++k) {
+                    float v = cl[k] + cr[j - k];
+                    if (v < best) {
+                        best = v;
+                        bk = (uint8_t)k;
+                    }
+                }
+                dist[j] = best;
+                dk[j] = bk;
+            }
+            float inner = A * P.cwbvh_node_cost + dist[8];
+            float leafc = nd.prims <= 3 ? A * nd.prims * P.cwbvh_tri_cost : INFINITY;
+            if (leafc <= inner) {
+                c[1] = leafc;
+                d[1] = 1;
+            } else {
+                c[1] = inner;
+                d[1] = 0;
+            }
+            split8[n] = dk[8];
+            for (int i = 2; i < 8; ++i) {
+                if (dist[i] < c[i - 1]) {
+                    c[i] = dist[i];
+                    d[i] = dk[i];
+                } else {
+                    c[i] = c[i - 1];
+                    d[i] = 0;
+                }
+            }
+        }
+    }
+
+    // children of the 8-wide node made from binary node n
+    void collect(int32_t n, int i, std::vector<int32_t>& out) const {
+        const BNode& nd = T.nodes[n];
+        while (i > 1 && dec[(size_t)n * 8 + i] == 0) --i;
+        if (i == 1 || nd.left < 0) {
+            out.push_back(n);
+            return;
+        }
+        int k = dec[(size_t)n * 8 + i];
+        collect(nd.left, k, out);
+        collect(nd.right, i - k, out);
+    }
+    void children_of(int32_t n, std::vector<int32_t>& out) const {
+        const BNode& nd = T.nodes[n];
+        if (nd.left < 0) {
+            out.push_back(n);
+            return;
+        }
+        int k = split8[n];
+        collect(nd.left, k, out);
+        collect(nd.right, 8 - k, out);
+    }
+    bool is_leaf(int32_t n) const { return T.nodes[n].left < 0 || dec[(size_t)n * 8 + 1] == 1; }
+};
+
+int exponent_for(double extent) {
+    if (!(extent > 0.0)) return -126;
+    int e = (int)std::ceil(std::log2(extent / 255.0));
+    if (e < -126) e = -126;
+    if (e > 127) e = 127;
+    while (e < 127 && std::ldexp(255.0, e) < extent) ++e;
+    while (e > -126 && std::ldexp(255.0, e - 1) >= extent) --e;
+    return e;
+}
+
+void assign_slots(const Box& nb, const std::vector<Box>& cbox, int slot_of[8]) {
+    int nc = (int)cbox.size();
+    V3 nc_c = nb.center();
+    struct Cand { float cost; int s, c; };
+    std::vector<Cand> cand;
+    for (int s = 0; s < 8; ++s) {
+        V3 ds{(s & 4) ? -1.f : 1.f, (s & 2) ? -1.f : 1.f, (s & 1) ? -1.f : 1.f};
+        for (int c = 0; c < nc; ++c) cand.push_back({dot(cbox[c].center() - nc_c, ds), s, c});
+    }
+    std::stable_sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) { return a.cost < b.cost; });
+    bool used[8] = {};
+    for (int c = 0; c < nc; ++c) slot_of[c] = -1;
+    int assigned = 0;
+    for (const Cand& x : cand) {
+        if (assigned == nc) break;
+        if (used[x.s] || slot_of[x.c] >= 0) continue;
+        used[x.s] = true;
+        slot_of[x.c] = x.s;
+        ++assigned;
+    }
+}
+
+void layout_cwbvh(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params& P, cwbvh_layout& L) {
+    Collapse CL(T, P);
+    if (!T.nodes.empty() && T.nodes[T.root].prims > 0) CL.run();
+    const int S = P.cwbvh_node_stride;
+    L.kind = CWBVH_KIND_CWBVH;
+    L.node_stride = S;
+    auto new_nodes = [&](int count) {
+        int32_t idx = (int32_t)(L.nodes.size() / S);
+        L.nodes.resize(L.nodes.size() + (size_t)count * S, cwbvh_float4{0.f, 0.f, 0.f, 0.f});
+        L.num_nodes += count;
+        return idx;
+    };
+    auto gather = [&](int32_t n, std::vector<int32_t>& out) {
+        std::vector<int32_t> st{n};
+        while (!st.empty()) {
+            int32_t x = st.back();
+            st.pop_back();
+            const BNode& b = T.nodes[x];
+            if (b.left < 0) {
+                for (int32_t k = 0; k < b.count; ++k) out.push_back(T.refs[b.first + k]);
+            } else {
+                st.push_back(b.right);
+                st.push_back(b.left);
+            }
+        }
+    };
+    double sah = 0.0;
+    double root_area = T.nodes.empty() ? 0.0 : T.nodes[T.root].box.area();
+
+    new_nodes(1);
+    if (T.nodes.empty() || T.nodes[T.root].prims == 0) {
+        L.max_depth = 1;
+        L.stack_need = 2;
+        L.sah_cost = 0.0;
+        return;
+    }
+    struct Item { int32_t n, node_index, depth; };
+    std::vector<Item> st{{T.root, 0, 1}};
+    int32_t maxd = 0;
+    std::vector<int32_t> kids;
+    std::vector<Box> cbox;
+    while (!st.empty()) {
+        Item it = st.back();
+        st.pop_back();
+        maxd = std::max(maxd, it.depth);
+        kids.clear();
+        CL.children_of(it.n, kids);
+        int nc = (int)kids.size();
+        cbox.clear();
+        Box nb;
+        for (int32_t k : kids) {
+            cbox.push_back(T.nodes[k].box);
+            nb.grow(T.nodes[k].box);
+        }
+        if (root_area > 0) sah += P.cwbvh_node_cost * nb.area() / root_area;
+        int slot_of[8];
+        assign_slots(nb, cbox, slot_of);
+        int32_t child_at[8];
+        for (int s = 0; s < 8; ++s) child_at[s] = -1;
+        for (int c = 0; c < nc; ++c) child_at[slot_of[c]] = c;
+
+        int ex[3];
+        for (int a = 0; a < 3; ++a) ex[a] = exponent_for((double)comp(nb.hi, a) - (double)comp(nb.lo, a));
+        uint8_t imask = 0;
+        int inner_count = 0, tri_count = 0;
+        int32_t child_base = 0, tri_base = 0;
+        // inner children are allocated contiguously, in slot order, before
+        // any pointer into the node array is taken
+        int n_inner = 0;
+        for (int s = 0; s < 8; ++s)
+            if (child_at[s] >= 0 && !CL.is_leaf(kids[child_at[s]])) ++n_inner;
+        if (n_inner) child_base = new_nodes(n_inner);
+        tri_base = (int32_t)(L.woop.size() / 3);
+        cwbvh_float4* node = &L.nodes[(size_t)it.node_index * S];
+        uint8_t* n0b = reinterpret_cast<uint8_t*>(&node[0]);
+        uint8_t* meta = reinterpret_cast<uint8_t*>(&node[1]) + 8;
+        uint8_t* q = reinterpret_cast<uint8_t*>(&node[2]);
+        node[0].x = nb.lo.x;
+        node[0].y = nb.lo.y;
+        node[0].z = nb.lo.z;
+        for (int s = 0; s < 8; ++s) {
+            int c = child_at[s];
+            if (c < 0) continue;
+            const Box& b = cbox[c];
+            for (int a = 0; a < 3; ++a) {
+                double scale = std::ldexp(1.0, -ex[a]);
+                double lo = std::floor(((double)comp(b.lo, a) - (double)comp(nb.lo, a)) * scale);
+                double hi = std::ceil(((double)comp(b.hi, a) - (double)comp(nb.lo, a)) * scale);
+                lo = std::min(255.0, std::max(0.0, lo));
+                hi = std::min(255.0, std::max(0.0, hi));
+                q[a * 8 + s] = (uint8_t)lo;
+                q[24 + a * 8 + s] = (uint8_t)hi;
+            }
+            int32_t bn = kids[c];
+            if (!CL.is_leaf(bn)) {
+                imask |= (uint8_t)(1u << s);
+                meta[s] = (uint8_t)((1u << 5) | (24u + (unsigned)s));
+                st.push_back({bn, child_base + inner_count, it.depth + 1});
+                ++inner_count;
+            } else {
+                std::vector<int32_t> prims;
+                gather(bn, prims);
+                int cnt = (int)prims.size();
+                unsigned unary = cnt == 1 ? 1u : (cnt == 2 ? 3u : 7u);
+                meta[s] = (uint8_t)((unary << 5) | (unsigned)tri_count);
+                tri_count += cnt;
+                for (int32_t p : prims) append_tri(mesh, p, L);
+                L.num_leaves++;
+                if (root_area > 0) sah += P.cwbvh_tri_cost * cnt * b.area() / root_area;
+            }
+        }
+        n0b[12] = (uint8_t)(int8_t)ex[0];
+        n0b[13] = (uint8_t)(int8_t)ex[1];
+        n0b[14] = (uint8_t)(int8_t)ex[2];
+        n0b[15] = imask;
+        node[1].x = i2f(child_base);
+        node[1].y = i2f(tri_base);
+    }
+    L.max_depth = maxd;
+    L.stack_need = maxd + 1;
+    L.sah_cost = sah;
+}
+
+bool valid_params(const cwbvh_build_params& P) {
+    return P.sah_bins >= 4 && P.sah_bins <= 256 && P.bvh2_max_leaf >= 1 && P.bvh2_max_leaf <= 32 &&
+           (P.cwbvh_node_stride == 5 || P.cwbvh_node_stride == 8) && P.max_depth >= 8 &&
+           P.max_depth <= CWBVH_BVH2_STACK_CAPACITY - 2 && P.bvh2_node_cost >= 0 && P.bvh2_tri_cost > 0 &&
+           P.cwbvh_node_cost >= 0 && P.cwbvh_tri_cost > 0;
+}
+
+}  // namespace
+
+namespace cw {
+void woopify_host(const float* v0, const float* v1, const float* v2, cwbvh_float4* out3) {
+    woopify(V3{v0[0], v0[1], v0[2]}, V3{v1[0], v1[1], v1[2]}, V3{v2[0], v2[1], v2[2]}, out3);
+}
+}  // namespace cw
+
+extern "C" {
+
+int32_t cwbvh_build_params_default(cwbvh_build_params* p) {
+    CW_CHECK_ARG(p != nullptr, "params is NULL");
+    p->num_threads = 0;
+    p->sah_bins = 32;
+    p->bvh2_node_cost = 1.0f;
+    p->bvh2_tri_cost = 1.0f;
+    p->bvh2_max_leaf = 8;
+    p->cwbvh_node_cost = 1.0f;
+    p->cwbvh_tri_cost = 0.3f;
+    p->cwbvh_node_stride = 5;
+    p->max_depth = 60;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_build_layouts(const cwbvh_mesh* mesh, const cwbvh_build_params* params, cwbvh_layout** bvh2_out,
+                            cwbvh_layout** cwbvh_out) {
+    CW_CHECK_ARG(mesh != nullptr, "mesh is NULL");
+    CW_CHECK_ARG(bvh2_out != nullptr || cwbvh_out != nullptr, "no output requested");
+    cwbvh_build_params P;
+    if (params) P = *params;
+    else cwbvh_build_params_default(&P);
+    CW_CHECK_ARG(valid_params(P), "invalid build parameters");
+    CW_CHECK_ARG(mesh->nt() < (int64_t)0x7FFFFFFF / 4, "too many triangles for 32-bit layouts");
+    Tree T;
+    build_tree(mesh, P, T);
+    if (bvh2_out) {
+        auto* L = new cwbvh_layout();
+        layout_bvh2(mesh, T, P, *L);
+        *bvh2_out = L;
+    }
+    if (cwbvh_out) {
+        auto* L = new cwbvh_layout();
+        layout_cwbvh(mesh, T, P, *L);
+        *cwbvh_out = L;
+    }
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_layout_from_arrays(int32_t kind, int32_t node_stride, const cwbvh_float4* nodes, int64_t nodes_f4,
+                                 const cwbvh_float4* woop, int64_t woop_f4, const int32_t* prim_idx,
+                                 int64_t prim_idx_len, cwbvh_layout** out) {
+    CW_CHECK_ARG(out != nullptr, "out is NULL");
+    CW_CHECK_ARG(kind == CWBVH_KIND_BVH2 || kind == CWBVH_KIND_CWBVH, "bad kind");
+    CW_CHECK_ARG(kind == CWBVH_KIND_BVH2 ? node_stride == 4 : (node_stride == 5 || node_stride == 8),
+                 "bad node stride");
+    CW_CHECK_ARG(nodes_f4 >= node_stride && nodes_f4 % node_stride == 0 && nodes, "bad node array");
+    CW_CHECK_ARG(woop_f4 >= 0 && (woop_f4 == 0 || woop), "bad woop array");
+    CW_CHECK_ARG(prim_idx_len >= 0 && (prim_idx_len == 0 || prim_idx), "bad prim index array");
+    auto* L = new cwbvh_layout();
+    L->kind = kind;
+    L->node_stride = node_stride;
+    L->nodes.assign(nodes, nodes + nodes_f4);
+    L->woop.assign(woop, woop + woop_f4);
+    L->prim_idx.assign(prim_idx, prim_idx + prim_idx_len);
+    L->num_nodes = nodes_f4 / node_stride;
+    // depth: walk the tree
+    int32_t maxd = 0;
+    if (kind == CWBVH_KIND_BVH2) {
+        std::vector<std::pair<int32_t, int32_t>> st{{0, 1}};
+        while (!st.empty()) {
+            auto [a, d] = st.back();
+            st.pop_back();
+            if (a < 0 || a + 3 >= nodes_f4 || d > 1024) { delete L; set_error("malformed BVH2 node array"); return CWBVH_ERR_INVALID_ARGUMENT; }
+            maxd = std::max(maxd, d);
+            int32_t c[2];
+            std::memcpy(&c[0], &L->nodes[a + 3].x, 4);
+            std::memcpy(&c[1], &L->nodes[a + 3].y, 4);
+            for (int k = 0; k < 2; ++k)
+                if (c[k] >= 0) st.push_back({c[k], d + 1});
+                else L->num_leaves++;
+        }
+        L->stack_need = maxd + 1;
+    } else {
+        std::vector<std::pair<int64_t, int32_t>> st{{0, 1}};
+        while (!st.empty()) {
+            auto [n, d] = st.back();
+            st.pop_back();
+            if (n < 0 || (n + 1) * node_stride > nodes_f4 || d > 1024) { delete L; set_error("malformed CWBVH node array"); return CWBVH_ERR_INVALID_ARGUMENT; }
+            maxd = std::max(maxd, d);
+            const uint8_t* n0b = reinterpret_cast<const uint8_t*>(&L->nodes[n * node_stride]);
+            int32_t base;
+            std::memcpy(&base, &L->nodes[n * node_stride + 1].x, 4);
+            uint8_t imask = n0b[15];
+            int k = 0;
+            for (int s = 0; s < 8; ++s)
+                if (imask & (1u << s)) st.push_back({(int64_t)base + k++, d + 1});
+        }
+        L->stack_need = maxd + 1;
+    }
+    L->max_depth = maxd;
+    for (int32_t v : L->prim_idx)
+        if (v >= 0) L->num_tri_refs++;
+    *out = L;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_layout_get_view(const cwbvh_layout* L, cwbvh_layout_view* v) {
+    CW_CHECK_ARG(L != nullptr && v != nullptr, "NULL argument");
+    v->kind = L->kind;
+    v->node_stride = L->node_stride;
+    v->nodes = L->nodes.data();
+    v->nodes_f4 = (int64_t)L->nodes.size();
+    v->woop = L->woop.data();
+    v->woop_f4 = (int64_t)L->woop.size();
+    v->prim_idx = L->prim_idx.data();
+    v->prim_idx_len = (int64_t)L->prim_idx.size();
+    v->max_depth = L->max_depth;
+    v->stack_need = L->stack_need;
+    v->num_nodes = L->num_nodes;
+    v->num_leaves = L->num_leaves;
+    v->num_tri_refs = L->num_tri_refs;
+    v->sah_cost = L->sah_cost;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_layout_restride(const cwbvh_layout* in, int32_t node_stride, cwbvh_layout** out) {
+    CW_CHECK_ARG(in != nullptr && out != nullptr, "NULL argument");
+    CW_CHECK_ARG(in->kind == CWBVH_KIND_CWBVH, "restride applies to CWBVH layouts");
+    CW_CHECK_ARG(node_stride == 5 || node_stride == 8, "node stride must be 5 or 8");
+    auto* L = new cwbvh_layout(*in);
+    L->node_stride = node_stride;
+    L->nodes.assign((size_t)in->num_nodes * node_stride, cwbvh_float4{0.f, 0.f, 0.f, 0.f});
+    for (int64_t n = 0; n < in->num_nodes; ++n)
+        for (int k = 0; k < 5; ++k) L->nodes[n * node_stride + k] = in->nodes[n * in->node_stride + k];
+    *out = L;
+    return CWBVH_OK;
+}
+
+void cwbvh_layout_free(cwbvh_layout* layout) { delete layout; }
+
+int32_t cwbvh_woopify(const float* v0, const float* v1, const float* v2, cwbvh_float4* out3) {
+    CW_CHECK_ARG(v0 && v1 && v2 && out3, "NULL argument");
+    cw::woopify_host(v0, v1, v2, out3);
+    return CWBVH_OK;
+}
+
+}  // extern "C"

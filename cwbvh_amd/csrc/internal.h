@@ -1,0 +1,89 @@
+// Host-side internal interfaces between the C ABI (api.cpp, compat.cpp,
+// batch/serving code) and the kernel launchers (*.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <mutex>
+#include <string>
+
+#include "common.h"
+
+struct cwbvh_scene {
+    int32_t kind = CWBVH_KIND_CWBVH;
+    int32_t node_stride = 5;
+    const void* nodes = nullptr;   // device
+    const void* woop = nullptr;    // device
+    const int32_t* prim_idx = nullptr;
+    int64_t woop_f4 = 0;           // valid float4 count of the Woop array (0 = unknown)
+    int32_t stack_need = 0;
+    bool owning = false;
+    int64_t bytes = 0;
+    int32_t device = 0;
+};
+
+namespace cw {
+
+// HIP error -> status + message
+int32_t hip_fail(hipError_t e, const char* what);
+#define CW_HIP(call)                                      \
+    do {                                                  \
+        hipError_t e_ = (call);                           \
+        if (e_ != hipSuccess) return ::cw::hip_fail(e_, #call); \
+    } while (0)
+
+// Per-stream work counters for persistent launches: uint32[4] =
+// {ray chunk counter, exited waves, stack overflow flag, spare}.  The kernel
+// resets [0] and [1] when its last wave exits, so launches on one stream can
+// reuse the same counters (and a captured graph replays cleanly).
+int32_t stream_counters(hipStream_t stream, unsigned** out);
+
+struct TraceLaunch {
+    const cwbvh_scene* scene;
+    const void* rays;
+    const int32_t* order;
+    void* hits;
+    int64_t n;
+    uint32_t flags;
+    cwbvh_trace_config cfg;
+    uint64_t* stats;
+    unsigned* counters;   // from stream_counters (or a dedicated buffer)
+    hipStream_t stream;
+};
+
+int32_t launch_cwbvh(const TraceLaunch& L);
+int32_t launch_bvh2(const TraceLaunch& L);
+
+// Fill the device grid for a persistent kernel: resident blocks per CU * CUs.
+int device_compute_units();
+
+// One multi-scene job for the persistent batched kernel.
+struct BatchJob {
+    const void* nodes;
+    const void* woop;
+    const void* rays;
+    const int32_t* order;   // may be null
+    void* hits;
+    const int32_t* prim_idx;
+    int32_t n;
+    int32_t node_stride;
+    uint32_t chunk_begin;   // first global chunk of this job
+    uint32_t pad;
+};
+static_assert(sizeof(BatchJob) == 64, "BatchJob is 64 bytes");
+
+struct BatchLaunch {
+    const BatchJob* d_jobs;
+    int32_t num_jobs;
+    uint32_t total_chunks;
+    uint32_t flags;
+    cwbvh_trace_config cfg;   // fetch_chunk is the job chunk size
+    uint64_t* stats;
+    unsigned* counters;
+    hipStream_t stream;
+};
+int32_t launch_cwbvh_batch(const BatchLaunch& L);
+
+int32_t validate_config(int32_t kind, const cwbvh_trace_config& c);
+
+}  // namespace cw

@@ -1,0 +1,604 @@
+// Traversal kernels for gfx950 (wave64) and their launchers.
+//
+//  * cwbvh_trace: persistent CWBVH traversal with wave-level work fetching.
+//    Follows the reference kernel src/TraversalKernelCWBVH.cu:24-375:
+//      - ray fetch: terminated lanes of a wave are refilled together; one
+//        lane claims `fetch_chunk` rays with a single atomic and the wave
+//        hands them out by prefix rank (ballot + popcount), instead of one
+//        atomic per refill as in the reference (:60-72);
+//      - node group / triangle group state machine, highest-bit-first child
+//        order with octant XOR, 8-child quantized slab test (:93-295);
+//      - optional triangle postponing (:297-311) and the dynamic-fetch exit
+//        when idle lanes accumulate (:365-371), both tunable for 64 lanes;
+//      - traversal stack: first `lds_depth` entries in LDS (interleaved by
+//        lane), the rest in per-lane scratch, capacity 64 entries with an
+//        overflow flag (the reference has 8 shared + 32 local entries and
+//        no check).
+//    One template serves a single scene (uniform scene pointers) and the
+//    multi-scene batch (per-lane scene pointers from a job table).
+//  * bvh2_trace: one thread per ray, Aila-Laine while-while traversal with one
+//    postponed leaf and a wave-wide "any lane wants nodes" exit, Woop leaves
+//    closed by a 0x80000000 terminator (src/TraversalKernelBVH2.cu:26-178).
+//
+// Both kernels use the arithmetic contract of device_common.h, so results are
+// bit-identical between kernel variants and to the scalar CPU restatement
+// used by the tests.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+
+#include "device_common.h"
+#include "internal.h"
+
+using namespace cwd;
+
+namespace {
+
+constexpr int kMaxBlock = 256;
+
+struct CwbvhParams {
+    const float4* __restrict__ nodes;
+    const float4* __restrict__ tris;
+    const int* __restrict__ prim_map;
+    const Ray4* __restrict__ rays;
+    const int* __restrict__ order;
+    float4* __restrict__ hits;
+    const cw::BatchJob* __restrict__ jobs;
+    unsigned* work;                 // [0] fetch counter, [1] exited waves, [2] overflow
+    unsigned long long* stats;      // {nodes, triangles}
+    int n;                          // rays (single) or total chunks (multi)
+    int num_jobs;
+    int stride;                     // float4 per node
+    int lds_depth;
+    int chunk;
+    int idle_lanes;
+    int idle_budget;
+    int postpone_pct;
+    int prim_id;
+    int total_waves;
+};
+
+// Quantized slab test of 4 children (one meta dword, one byte per child).
+__device__ __forceinline__ unsigned child_hits4(unsigned meta4, unsigned lox, unsigned loy, unsigned loz,
+                                                unsigned hix, unsigned hiy, unsigned hiz, float adjx, float adjy,
+                                                float adjz, float origx, float origy, float origz, float tmin,
+                                                float tmax, unsigned octinv4) {
+    const unsigned is_inner4 = (meta4 & (meta4 << 1)) & 0x10101010u;
+    const unsigned ones = is_inner4 >> 4;                 // 0x01 per inner child byte
+    const unsigned inner_mask4 = (ones << 8) - ones;      // 0xFF per inner child byte
+    const unsigned bit_index4 = (meta4 ^ (octinv4 & inner_mask4)) & 0x1F1F1F1Fu;
+    const unsigned child_bits4 = (meta4 >> 5) & 0x07070707u;
+    unsigned hitmask = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int sh = 8 * j;
+        const float tnx = __builtin_fmaf((float)((lox >> sh) & 0xFFu), adjx, origx);
+        const float tny = __builtin_fmaf((float)((loy >> sh) & 0xFFu), adjy, origy);
+        const float tnz = __builtin_fmaf((float)((loz >> sh) & 0xFFu), adjz, origz);
+        const float tfx = __builtin_fmaf((float)((hix >> sh) & 0xFFu), adjx, origx);
+        const float tfy = __builtin_fmaf((float)((hiy >> sh) & 0xFFu), adjy, origy);
+        const float tfz = __builtin_fmaf((float)((hiz >> sh) & 0xFFu), adjz, origz);
+        const float cmin = fmaxf(imax3(tnx, tny, tnz), tmin);
+        const float cmax = fminf(imin3(tfx, tfy, tfz), tmax);
+        const unsigned bits = (child_bits4 >> sh) & 0xFFu;
+        const unsigned idx = (bit_index4 >> sh) & 0xFFu;
+        hitmask |= (cmin <= cmax) ? (bits << idx) : 0u;
+    }
+    return hitmask;
+}
+
+template <bool ANY, bool STATS, bool MULTI>
+__global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
+    extern __shared__ uint2 lds_stack[];
+    Stack<uint2, kStackCapacity> stack;
+    stack.init(lds_stack, threadIdx.x, blockDim.x, P.lds_depth);
+    const int lane = lane_id();
+
+    // per-lane ray state
+    int out_idx = -1;
+    const float4* nodes = P.nodes;
+    const float4* tris = P.tris;
+    float4* hits = P.hits;
+    const int* prim_map = P.prim_map;
+    int stride = P.stride;
+    float3 o = make_float3(0.f, 0.f, 0.f), d = o;
+    float tmin = 0.f, tmax = 0.f, idx = 0.f, idy = 0.f, idz = 0.f;
+    unsigned octinv = 0;
+    uint2 ng = make_uint2(0u, 0u), tg = make_uint2(0u, 0u);
+    int sp = 0;
+    int hit_addr = -1;
+    float hu = 0.f, hv = 0.f;
+    unsigned long long st_nodes = 0, st_tris = 0;
+
+    // wave-uniform work pool
+    unsigned pool_next = 0, pool_end = 0;
+    int pool_job = 0;
+    bool exhausted = false;
+
+    while (true) {
+        const bool terminated = sp == 0 && ng.y <= 0x00FFFFFFu && tg.y == 0u;
+        unsigned long long need = __ballot(terminated);
+        bool got = false;
+        unsigned my = 0;
+        int my_job = 0;
+        while (need != 0ull) {
+            if (pool_next >= pool_end) {
+                if (exhausted) break;
+                const int leader = __ffsll((long long)need) - 1;
+                unsigned base = 0;
+                if (lane == leader) base = atomicAdd(&P.work[0], MULTI ? 1u : (unsigned)P.chunk);
+                base = __shfl(base, leader);
+                if (base >= (unsigned)P.n) {
+                    exhausted = true;
+                    break;
+                }
+                if (MULTI) {
+                    // job j owns chunks [jobs[j].chunk_begin, jobs[j+1].chunk_begin)
+                    int lo = 0, hi = P.num_jobs - 1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (P.jobs[mid].chunk_begin <= base) lo = mid;
+                        else hi = mid - 1;
+                    }
+                    pool_job = lo;
+                    const unsigned first = (base - P.jobs[lo].chunk_begin) * (unsigned)P.chunk;
+                    pool_next = first;
+                    pool_end = min(first + (unsigned)P.chunk, (unsigned)P.jobs[lo].n);
+                } else {
+                    pool_next = base;
+                    pool_end = min(base + (unsigned)P.chunk, (unsigned)P.n);
+                }
+            }
+            const unsigned avail = pool_end - pool_next;
+            const unsigned rank = (unsigned)__popcll(need & lanemask_lt());
+            if (((need >> lane) & 1ull) && rank < avail) {
+                my = pool_next + rank;
+                my_job = pool_job;
+                got = true;
+            }
+            pool_next += min((unsigned)__popcll(need), avail);
+            need = __ballot(terminated && !got);
+        }
+        if (terminated && !got) break;  // no work left for this lane
+
+        if (got) {
+            const Ray4* rays = P.rays;
+            const int* order = P.order;
+            if (MULTI) {
+                const cw::BatchJob& J = P.jobs[my_job];
+                nodes = (const float4*)J.nodes;
+                tris = (const float4*)J.woop;
+                hits = (float4*)J.hits;
+                prim_map = J.prim_idx;
+                stride = J.node_stride;
+                rays = (const Ray4*)J.rays;
+                order = J.order;
+            }
+            const int ri = order ? order[my] : (int)my;
+            out_idx = ri;
+            const float4 ro = rays[ri].o, rd = rays[ri].d;
+            o = make_float3(ro.x, ro.y, ro.z);
+            d = make_float3(rd.x, rd.y, rd.z);
+            tmin = ro.w;
+            tmax = rd.w;
+            idx = safe_rcp(d.x);
+            idy = safe_rcp(d.y);
+            idz = safe_rcp(d.z);
+            const unsigned oct = ((d.x < 0.f ? 1u : 0u) << 2) | ((d.y < 0.f ? 1u : 0u) << 1) | (d.z < 0.f ? 1u : 0u);
+            octinv = 7u - oct;
+            ng = make_uint2(0u, 0x80000000u);
+            tg = make_uint2(0u, 0u);
+            sp = 0;
+            hit_addr = -1;
+            hu = hv = 0.f;
+        }
+
+        int lost = 0;
+        while (true) {
+            if (ng.y > 0x00FFFFFFu) {
+                const unsigned imask = ng.y;
+                const int bit = 31 - __clz((int)ng.y);
+                const unsigned base = ng.x;
+                ng.y &= ~(1u << bit);
+                if (ng.y > 0x00FFFFFFu) {
+                    if (sp < kStackCapacity) {
+                        stack.put(sp, ng);
+                        ++sp;
+                    } else {
+                        P.work[2] = 1u;
+                    }
+                }
+                const unsigned slot = (unsigned)(bit - 24) ^ octinv;
+                const unsigned rel = (unsigned)__popc(imask & ~(0xFFFFFFFFu << slot));
+                const float4* np = nodes + (size_t)(base + rel) * (unsigned)stride;
+                const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
+                if (STATS) ++st_nodes;
+                const unsigned eb = f2u(n0.w);
+                const int ex = (int)(signed char)(eb & 0xFFu);
+                const int ey = (int)(signed char)((eb >> 8) & 0xFFu);
+                const int ez = (int)(signed char)((eb >> 16) & 0xFFu);
+                const float adjx = u2f((unsigned)(ex + 127) << 23) * idx;
+                const float adjy = u2f((unsigned)(ey + 127) << 23) * idy;
+                const float adjz = u2f((unsigned)(ez + 127) << 23) * idz;
+                const float origx = -(o.x - n0.x) * idx;
+                const float origy = -(o.y - n0.y) * idy;
+                const float origz = -(o.z - n0.z) * idz;
+                const unsigned octinv4 = octinv * 0x01010101u;
+                const bool sx = idx < 0.f, sy = idy < 0.f, sz = idz < 0.f;
+                unsigned hitmask;
+                {
+                    const unsigned qlx = f2u(n2.x), qly = f2u(n2.z), qlz = f2u(n3.x);
+                    const unsigned qhx = f2u(n3.z), qhy = f2u(n4.x), qhz = f2u(n4.z);
+                    hitmask = child_hits4(f2u(n1.z), sx ? qhx : qlx, sy ? qhy : qly, sz ? qhz : qlz,
+                                          sx ? qlx : qhx, sy ? qly : qhy, sz ? qlz : qhz, adjx, adjy, adjz,
+                                          origx, origy, origz, tmin, tmax, octinv4);
+                }
+                {
+                    const unsigned qlx = f2u(n2.y), qly = f2u(n2.w), qlz = f2u(n3.y);
+                    const unsigned qhx = f2u(n3.w), qhy = f2u(n4.y), qhz = f2u(n4.w);
+                    hitmask |= child_hits4(f2u(n1.w), sx ? qhx : qlx, sy ? qhy : qly, sz ? qhz : qlz,
+                                           sx ? qlx : qhx, sy ? qly : qhy, sz ? qlz : qhz, adjx, adjy, adjz,
+                                           origx, origy, origz, tmin, tmax, octinv4);
+                }
+                ng.x = f2u(n1.x);
+                ng.y = (hitmask & 0xFF000000u) | (eb >> 24);
+                tg.x = f2u(n1.y);
+                tg.y = hitmask & 0x00FFFFFFu;
+            } else {
+                tg = ng;
+                ng = make_uint2(0u, 0u);
+            }
+
+            bool finished = false;
+            const int total = P.postpone_pct > 0 ? __popcll(__ballot(1)) : 0;
+            while (tg.y != 0u) {
+                if (P.postpone_pct > 0) {
+                    const int active = __popcll(__ballot(1));
+                    if (active * 100 < total * P.postpone_pct) {
+                        if (sp < kStackCapacity) {
+                            stack.put(sp, tg);
+                            ++sp;
+                        } else {
+                            P.work[2] = 1u;
+                        }
+                        tg.y = 0u;
+                        break;
+                    }
+                }
+                const int ti = 31 - __clz((int)tg.y);
+                const int addr = (int)(tg.x * 3u + (unsigned)ti * 3u);
+                const float4 v0 = tris[addr], v1 = tris[addr + 1], v2 = tris[addr + 2];
+                if (STATS) ++st_tris;
+                float t, u, v;
+                if (woop_test<ANY>(v0, v1, v2, o, d, tmin, tmax, addr, hit_addr, t, u, v)) {
+                    tmax = t;
+                    hit_addr = addr;
+                    hu = u;
+                    hv = v;
+                    if (ANY) {
+                        finished = true;
+                        break;
+                    }
+                }
+                tg.y &= ~(1u << ti);
+            }
+
+            if (!finished && ng.y <= 0x00FFFFFFu) {
+                if (sp > 0) {
+                    --sp;
+                    ng = stack.get(sp);
+                } else {
+                    finished = true;
+                }
+            }
+            if (finished) {
+                const int out_id = (hit_addr >= 0 && P.prim_id) ? prim_map[hit_addr] : hit_addr;
+                hits[out_idx] = make_float4(tmax, i2f(out_id), hu, hv);
+                sp = 0;
+                ng = make_uint2(0u, 0u);
+                tg = make_uint2(0u, 0u);
+                break;
+            }
+            lost += __popcll(~__ballot(1)) - P.idle_lanes;
+            if (lost >= P.idle_budget) break;
+        }
+    }
+
+    if (STATS) {
+        atomicAdd(&P.stats[0], st_nodes);
+        atomicAdd(&P.stats[1], st_tris);
+    }
+    // last wave out resets the fetch counters for the next launch on this stream
+    __threadfence();
+    const unsigned long long alive = __ballot(1);
+    if (lane == __ffsll((long long)alive) - 1) {
+        const unsigned prev = atomicAdd(&P.work[1], 1u);
+        if (prev == (unsigned)P.total_waves - 1u) {
+            atomicExch(&P.work[0], 0u);
+            atomicExch(&P.work[1], 0u);
+        }
+    }
+}
+
+struct Bvh2Params {
+    const float4* __restrict__ nodes;
+    const float4* __restrict__ tris;
+    const int* __restrict__ prim_map;
+    const Ray4* __restrict__ rays;
+    const int* __restrict__ order;
+    float4* __restrict__ hits;
+    unsigned* work;
+    unsigned long long* stats;
+    int n;
+    int lds_depth;
+    int woop_last;   // last valid float4 index of tris (loads past a leaf terminator clamp here)
+    int prim_id;
+};
+
+// Slab test of one BVH2 child box (reference src/TraversalKernelBVH2.cu:14-24).
+__device__ __forceinline__ bool box2(float lox, float hix, float loy, float hiy, float loz, float hiz, float idx,
+                                     float idy, float idz, float oodx, float oody, float oodz, float tmin, float tmax,
+                                     float& dist) {
+    const float lx = __builtin_fmaf(lox, idx, -oodx), hx = __builtin_fmaf(hix, idx, -oodx);
+    const float ly = __builtin_fmaf(loy, idy, -oody), hy = __builtin_fmaf(hiy, idy, -oody);
+    const float lz = __builtin_fmaf(loz, idz, -oodz), hz = __builtin_fmaf(hiz, idz, -oodz);
+    const float smin = imax2(imax3(imin2(lx, hx), imin2(ly, hy), imin2(lz, hz)), tmin);
+    const float smax = imin2(imin3(imax2(lx, hx), imax2(ly, hy), imax2(lz, hz)), tmax);
+    dist = smin;
+    return smin <= smax;
+}
+
+template <bool ANY, bool STATS>
+__global__ __launch_bounds__(kMaxBlock) void bvh2_trace(Bvh2Params P) {
+    constexpr int SENTINEL = 0x76543210;
+    extern __shared__ int lds_stack_i[];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P.n) return;
+    Stack<int, kBvh2StackCapacity> stack;
+    stack.init(lds_stack_i, threadIdx.x, blockDim.x, P.lds_depth);
+
+    const int ri = P.order ? P.order[i] : i;
+    const float4 ro = P.rays[ri].o, rd = P.rays[ri].d;
+    const float3 o = make_float3(ro.x, ro.y, ro.z), d = make_float3(rd.x, rd.y, rd.z);
+    const float tmin = ro.w;
+    float hitT = rd.w;
+    const float idx = safe_rcp(d.x), idy = safe_rcp(d.y), idz = safe_rcp(d.z);
+    const float oodx = o.x * idx, oody = o.y * idy, oodz = o.z * idz;
+    unsigned long long st_nodes = 0, st_tris = 0;
+
+    int sp = 0;
+    stack.put(0, SENTINEL);
+    int node = 0, leaf = 0, hit_addr = -1;
+    float hu = 0.f, hv = 0.f;
+    bool done = false;
+    while (node != SENTINEL) {
+        leaf = 0;
+        while (node != SENTINEL && node >= 0) {
+            const float4* np = P.nodes + node;
+            const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+            if (STATS) ++st_nodes;
+            float d0, d1;
+            const bool t0 = box2(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, idx, idy, idz, oodx, oody, oodz, tmin, hitT, d0);
+            const bool t1 = box2(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, idx, idy, idz, oodx, oody, oodz, tmin, hitT, d1);
+            const int c0 = f2i(n3.x), c1 = f2i(n3.y);
+            if (!t0 && !t1) {
+                node = stack.get(sp);
+                --sp;
+            } else {
+                node = t0 ? c0 : c1;
+                if (t0 && t1) {
+                    int far = c1;
+                    if (d1 < d0) {
+                        far = node;
+                        node = c1;
+                    }
+                    if (sp + 1 < kBvh2StackCapacity) {
+                        ++sp;
+                        stack.put(sp, far);
+                    } else {
+                        P.work[2] = 1u;
+                    }
+                }
+            }
+            if (node < 0 && leaf >= 0) {
+                leaf = node;
+                node = stack.get(sp);
+                --sp;
+            }
+            if (!__any(leaf >= 0)) break;
+        }
+        while (leaf < 0) {
+            for (int a = ~leaf;; a += 3) {
+                const float4 v0 = P.tris[a];
+                const float4 v1 = P.tris[min(a + 1, P.woop_last)];
+                const float4 v2 = P.tris[min(a + 2, P.woop_last)];
+                if (f2u(v0.x) == 0x80000000u) break;
+                if (STATS) ++st_tris;
+                float t, u, v;
+                if (woop_test<ANY>(v0, v1, v2, o, d, tmin, hitT, a, hit_addr, t, u, v)) {
+                    hitT = t;
+                    hit_addr = a;
+                    hu = u;
+                    hv = v;
+                    if (ANY) {
+                        done = true;
+                        break;
+                    }
+                }
+            }
+            if (done) break;
+            leaf = node;
+            if (node < 0) {
+                node = stack.get(sp);
+                --sp;
+            }
+        }
+        if (done) break;
+    }
+    const int out_id = (hit_addr >= 0 && P.prim_id) ? P.prim_map[hit_addr] : hit_addr;
+    P.hits[ri] = make_float4(hitT, i2f(out_id), hu, hv);
+    if (STATS) {
+        atomicAdd(&P.stats[0], st_nodes);
+        atomicAdd(&P.stats[1], st_tris);
+    }
+}
+
+__global__ void copy_u4_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, long long n) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long step = (long long)gridDim.x * blockDim.x;
+    for (; i < n; i += step) dst[i] = src[i];
+}
+
+// ------------------------------------------------------------ occupancy
+
+template <class K>
+int resident_blocks(K kernel, int block, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, size_t>, int> cache;
+    std::lock_guard<std::mutex> g(mu);
+    auto key = std::make_tuple((const void*)kernel, block, lds);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, block, lds) != hipSuccess || nb < 1) nb = 1;
+    cache[key] = nb;
+    return nb;
+}
+
+template <bool ANY, bool STATS, bool MULTI>
+int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t stream) {
+    const int block = cfg.block_size;
+    const size_t lds = (size_t)block * (size_t)cfg.lds_depth * sizeof(uint2);
+    auto kernel = cwbvh_trace<ANY, STATS, MULTI>;
+    int grid = cfg.grid_blocks;
+    if (grid <= 0) grid = resident_blocks(kernel, block, lds) * cw::device_compute_units();
+    p.total_waves = grid * (block / 64);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, p);
+    CW_HIP(hipGetLastError());
+    return CWBVH_OK;
+}
+
+template <bool ANY, bool STATS>
+int32_t launch_bvh2_t(Bvh2Params& p, const cwbvh_trace_config& cfg, hipStream_t stream) {
+    const int block = cfg.block_size;
+    const size_t lds = (size_t)block * (size_t)cfg.lds_depth * sizeof(int);
+    const long long grid = ((long long)p.n + block - 1) / block;
+    hipLaunchKernelGGL((bvh2_trace<ANY, STATS>), dim3((unsigned)grid), dim3(block), lds, stream, p);
+    CW_HIP(hipGetLastError());
+    return CWBVH_OK;
+}
+
+}  // namespace
+
+namespace cw {
+
+int32_t validate_config(int32_t kind, const cwbvh_trace_config& c) {
+    if (!(c.block_size == 64 || c.block_size == 128 || c.block_size == 256)) {
+        set_error("block_size must be 64, 128 or 256");
+        return CWBVH_ERR_INVALID_ARGUMENT;
+    }
+    const int cap = kind == CWBVH_KIND_CWBVH ? kStackCapacity : kBvh2StackCapacity;
+    if (c.lds_depth < 1 || c.lds_depth > cap) {
+        set_error("lds_depth must be in [1, stack capacity]");
+        return CWBVH_ERR_INVALID_ARGUMENT;
+    }
+    const size_t lds = (size_t)c.block_size * c.lds_depth * (kind == CWBVH_KIND_CWBVH ? 8 : 4);
+    if (lds > 160 * 1024) {
+        set_error("LDS stack exceeds 160 KiB per block");
+        return CWBVH_ERR_INVALID_ARGUMENT;
+    }
+    if (kind == CWBVH_KIND_CWBVH) {
+        if (c.fetch_chunk < 1 || c.fetch_chunk > (1 << 20) || c.grid_blocks < 0 || c.refill_budget < 1 ||
+            c.refill_idle < 0 || c.refill_idle > 64 || c.postpone_pct < 0 || c.postpone_pct > 100 ||
+            c.prefetch < 0 || c.prefetch > 1) {
+            set_error("invalid CWBVH trace configuration");
+            return CWBVH_ERR_INVALID_ARGUMENT;
+        }
+    }
+    return CWBVH_OK;
+}
+
+int32_t launch_cwbvh(const TraceLaunch& L) {
+    const cwbvh_scene* s = L.scene;
+    CwbvhParams p{};
+    p.nodes = (const float4*)s->nodes;
+    p.tris = (const float4*)s->woop;
+    p.prim_map = s->prim_idx;
+    p.rays = (const Ray4*)L.rays;
+    p.order = L.order;
+    p.hits = (float4*)L.hits;
+    p.jobs = nullptr;
+    p.work = L.counters;
+    p.stats = (unsigned long long*)L.stats;
+    p.n = (int)L.n;
+    p.num_jobs = 0;
+    p.stride = s->node_stride;
+    p.lds_depth = L.cfg.lds_depth;
+    p.chunk = L.cfg.fetch_chunk;
+    p.idle_lanes = L.cfg.refill_idle;
+    p.idle_budget = L.cfg.refill_budget;
+    p.postpone_pct = L.cfg.postpone_pct;
+    p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
+    const bool any = (L.flags & CWBVH_TRACE_ANY) != 0, stats = (L.flags & CWBVH_TRACE_STATS) != 0;
+    if (any) return stats ? launch_cwbvh_t<true, true, false>(p, L.cfg, L.stream)
+                          : launch_cwbvh_t<true, false, false>(p, L.cfg, L.stream);
+    return stats ? launch_cwbvh_t<false, true, false>(p, L.cfg, L.stream)
+                 : launch_cwbvh_t<false, false, false>(p, L.cfg, L.stream);
+}
+
+int32_t launch_cwbvh_batch(const BatchLaunch& L) {
+    CwbvhParams p{};
+    p.jobs = L.d_jobs;
+    p.num_jobs = L.num_jobs;
+    p.n = (int)L.total_chunks;
+    p.work = L.counters;
+    p.stats = (unsigned long long*)L.stats;
+    p.lds_depth = L.cfg.lds_depth;
+    p.chunk = L.cfg.fetch_chunk;
+    p.idle_lanes = L.cfg.refill_idle;
+    p.idle_budget = L.cfg.refill_budget;
+    p.postpone_pct = L.cfg.postpone_pct;
+    p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
+    const bool any = (L.flags & CWBVH_TRACE_ANY) != 0, stats = (L.flags & CWBVH_TRACE_STATS) != 0;
+    if (any) return stats ? launch_cwbvh_t<true, true, true>(p, L.cfg, L.stream)
+                          : launch_cwbvh_t<true, false, true>(p, L.cfg, L.stream);
+    return stats ? launch_cwbvh_t<false, true, true>(p, L.cfg, L.stream)
+                 : launch_cwbvh_t<false, false, true>(p, L.cfg, L.stream);
+}
+
+int32_t launch_bvh2(const TraceLaunch& L) {
+    const cwbvh_scene* s = L.scene;
+    Bvh2Params p{};
+    p.nodes = (const float4*)s->nodes;
+    p.tris = (const float4*)s->woop;
+    p.prim_map = s->prim_idx;
+    p.rays = (const Ray4*)L.rays;
+    p.order = L.order;
+    p.hits = (float4*)L.hits;
+    p.work = L.counters;
+    p.stats = (unsigned long long*)L.stats;
+    p.n = (int)L.n;
+    p.lds_depth = L.cfg.lds_depth;
+    p.woop_last = s->woop_f4 > 0 ? (int)(s->woop_f4 - 1) : 0x7FFFFFFF;
+    p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
+    const bool any = (L.flags & CWBVH_TRACE_ANY) != 0, stats = (L.flags & CWBVH_TRACE_STATS) != 0;
+    if (any) return stats ? launch_bvh2_t<true, true>(p, L.cfg, L.stream) : launch_bvh2_t<true, false>(p, L.cfg, L.stream);
+    return stats ? launch_bvh2_t<false, true>(p, L.cfg, L.stream) : launch_bvh2_t<false, false>(p, L.cfg, L.stream);
+}
+
+}  // namespace cw
+
+extern "C" int32_t cwbvh_copy_u4(void* dst, const void* src, int64_t n_u4, void* stream) {
+    CW_CHECK_ARG(n_u4 >= 0 && (n_u4 == 0 || (dst && src)), "bad copy arguments");
+    if (n_u4 == 0) return CWBVH_OK;
+    const int block = 256;
+    long long grid = (n_u4 + block - 1) / block;
+    if (grid > 65536) grid = 65536;
+    hipLaunchKernelGGL(copy_u4_kernel, dim3((unsigned)grid), dim3(block), 0, (hipStream_t)stream, (uint4*)dst,
+                       (const uint4*)src, (long long)n_u4);
+    CW_HIP(hipGetLastError());
+    return CWBVH_OK;
+}

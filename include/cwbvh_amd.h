@@ -1,0 +1,309 @@
+/*
+ * cwbvh_amd.h -- C ABI of the MI355X (gfx950) ray-traversal engine.
+ *
+ * Two traversal paths over triangle meshes, each with closest-hit and any-hit:
+ *   - CWBVH: 8-wide compressed wide BVH (quantized child boxes, octant-ordered
+ *     child visits), persistent wave64 kernel with work fetching;
+ *   - BVH2:  Aila-Laine while-while traversal with Woop triangle tests.
+ *
+ * Data contract (identical bytes to the reference engine):
+ *   ray  = float4(origin.xyz, tmin), float4(dir.xyz, tmax)     32 B
+ *          (reference: src/helper_math.h:1853-1857)
+ *   hit  = float4(t, triAddr as int32 bits, u, v)               16 B
+ *          (reference: src/helper_math.h:1859-1865)
+ *          miss: t = ray tmax, triAddr = -1, u = v = 0.
+ *          triAddr is the float4 index of the hit triangle's first Woop row;
+ *          prim id = prim_idx[triAddr] (or written directly with
+ *          CWBVH_TRACE_PRIM_ID).
+ *   BVH2 node  = 4 float4 (64 B)     (reference: src/GPUBVHConverter.cpp:10-121)
+ *   CWBVH node = 5 float4 (80 B), optionally padded to 8 float4 (128 B, one
+ *                HBM/L2 line per node)   (reference: src/GPUBVHConverter.cpp:123-378)
+ *   Woop triangle = 3 float4 (48 B)  (reference: src/GPUBVHConverter.cpp:380-403)
+ *
+ * Every function returns a cwbvh_status (0 = OK) unless stated otherwise.
+ * Device pointers are plain HIP device addresses; "stream" arguments are
+ * hipStream_t passed as void* (NULL = the null stream).
+ */
+#ifndef CWBVH_AMD_H
+#define CWBVH_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CWBVH_ABI_VERSION 1
+
+/* ------------------------------------------------------------------ types */
+
+typedef struct cwbvh_float4 { float x, y, z, w; } cwbvh_float4;
+
+typedef struct cwbvh_ray {
+    cwbvh_float4 origin_tmin;
+    cwbvh_float4 dir_tmax;
+} cwbvh_ray;
+
+typedef struct cwbvh_hit {
+    cwbvh_float4 t_triId_u_v;
+} cwbvh_hit;
+
+typedef enum cwbvh_status {
+    CWBVH_OK = 0,
+    CWBVH_ERR_INVALID_ARGUMENT = 1,
+    CWBVH_ERR_HIP = 2,
+    CWBVH_ERR_OUT_OF_MEMORY = 3,
+    CWBVH_ERR_STACK_DEPTH = 4,     /* tree deeper than the traversal stack */
+    CWBVH_ERR_NOT_BOUND = 5,       /* compat trace before bind */
+    CWBVH_ERR_NO_DEVICE = 6,
+    CWBVH_ERR_BUDGET = 7,          /* request larger than the memory budget */
+    CWBVH_ERR_CANCELLED = 8,
+    CWBVH_ERR_INTERNAL = 9
+} cwbvh_status;
+
+enum {
+    CWBVH_KIND_BVH2 = 0,
+    CWBVH_KIND_CWBVH = 1
+};
+
+/* trace flags */
+enum {
+    CWBVH_TRACE_CLOSEST = 0,      /* closest hit */
+    CWBVH_TRACE_ANY = 1,          /* any hit: stop at the first accepted triangle */
+    CWBVH_TRACE_PRIM_ID = 2,      /* write prim_idx[triAddr] instead of triAddr */
+    CWBVH_TRACE_STATS = 4         /* count visited nodes / tested triangles */
+};
+
+/* Largest traversal stack (entries) the kernels carry: LDS part + scratch spill. */
+#define CWBVH_STACK_CAPACITY 64
+#define CWBVH_BVH2_STACK_CAPACITY 64
+
+/* ------------------------------------------------------------ diagnostics */
+
+int32_t cwbvh_abi_version(void);
+const char* cwbvh_status_string(int32_t status);
+/* Detail message of the last failing call on this host thread. */
+const char* cwbvh_last_error(void);
+
+/* ------------------------------------------------------- meshes (host) --- */
+
+typedef struct cwbvh_mesh cwbvh_mesh;
+
+/* Copy caller arrays: vertices = float[3*nv], indices = int32[3*nt]. */
+int32_t cwbvh_mesh_from_arrays(const float* vertices, int64_t num_vertices,
+                               const int32_t* indices, int64_t num_triangles,
+                               cwbvh_mesh** out);
+int32_t cwbvh_mesh_view(const cwbvh_mesh* mesh, const float** vertices, int64_t* num_vertices,
+                        const int32_t** indices, int64_t* num_triangles);
+/* Axis-aligned bounds: lo[3], hi[3]. */
+int32_t cwbvh_mesh_bounds(const cwbvh_mesh* mesh, float* lo3, float* hi3);
+void cwbvh_mesh_free(cwbvh_mesh* mesh);
+
+/* Seeded heightfield over [0,1]^2: nx*ny cells, 2 triangles each, height =
+ * sum_{k<octaves} amp0*2^-k * sin(2*pi*2^k*(x cos th_k + y sin th_k) + ph_k). */
+int32_t cwbvh_mesh_heightfield(int32_t nx, int32_t ny, int32_t octaves, float amp0,
+                               uint64_t seed, cwbvh_mesh** out);
+/* gx*gy*gz randomly rotated geodesic spheres (20*freq^2 triangles each) with
+ * radius U[scale_min, scale_max] on a lattice of pitch `spacing`, plus an
+ * optional 2-triangle ground plane below the lattice. */
+int32_t cwbvh_mesh_icosphere_grid(int32_t gx, int32_t gy, int32_t gz, int32_t frequency,
+                                  float spacing, float scale_min, float scale_max,
+                                  int32_t ground_plane, uint64_t seed, cwbvh_mesh** out);
+/* Triangle soup in Gaussian clusters inside the unit cube: cluster sigma
+ * log-uniform in [sigma_min, sigma_max], edge lengths lognormal(mu, s). */
+int32_t cwbvh_mesh_cluster_soup(int64_t num_triangles, int32_t num_clusters,
+                                float sigma_min, float sigma_max, float edge_mu,
+                                float edge_sigma, uint64_t seed, cwbvh_mesh** out);
+/* Known-answer scene: triangle i lies in plane z = i with vertices
+ * (-1.1,-1.1,i), (1.1,-1.1,i), (1.1,1.1,i). */
+int32_t cwbvh_mesh_validation_planes(int32_t count, cwbvh_mesh** out);
+
+/* --------------------------------------------------------- rays (host) --- */
+
+/* AO rays from area-uniform surface points: num_points * rays_per_point rays,
+ * origin offset `offset` along the geometric normal (front side = the side the
+ * normal (v1-v0)x(v2-v0) points to), cosine-weighted directions. */
+int32_t cwbvh_rays_ao_surface(const cwbvh_mesh* mesh, int64_t num_points,
+                              int32_t rays_per_point, float offset, float tmin,
+                              float tmax, uint64_t seed, cwbvh_ray* out);
+/* Pinhole camera rays, row-major width*height, vertical fov in degrees. */
+int32_t cwbvh_rays_pinhole(int32_t width, int32_t height, const float* eye3,
+                           const float* target3, const float* up3, float fov_deg,
+                           float tmin, float tmax, cwbvh_ray* out);
+/* AO rays from primary hits: for every hit (prim id >= 0), rays_per_hit
+ * cosine-weighted rays around the geometric normal facing the incoming ray.
+ * prim_ids[i] is the mesh triangle of hit i (-1 = miss).  out must hold
+ * n*rays_per_hit rays; *out_count receives the number written. */
+int32_t cwbvh_rays_ao_from_hits(const cwbvh_mesh* mesh, const cwbvh_ray* primary,
+                                const cwbvh_hit* hits, const int32_t* prim_ids, int64_t n,
+                                int32_t rays_per_hit, float offset, float tmax,
+                                uint64_t seed, cwbvh_ray* out, int64_t* out_count);
+/* Diffuse secondary rays: area-uniform surface points, uniform sphere
+ * directions, origin offset along the normal on the side of the direction. */
+int32_t cwbvh_rays_sphere_from_surface(const cwbvh_mesh* mesh, int64_t num_rays, float offset,
+                                       float tmin, float tmax, uint64_t seed, cwbvh_ray* out);
+/* Known-answer rays for cwbvh_mesh_validation_planes(num_triangles): ray r
+ * starts at (0.5, -0.5, k - 0.5), k = (r * 12345678) mod num_triangles computed
+ * in 64-bit, direction normalize(-0.01, 0.5, 1), tmin 0, tmax 1e20. */
+int32_t cwbvh_rays_validation(int64_t count, int32_t num_triangles, cwbvh_ray* out);
+/* Fisher-Yates shuffle of a ray array (seeded). */
+int32_t cwbvh_rays_shuffle(cwbvh_ray* rays, int64_t n, uint64_t seed);
+
+/* ----------------------------------------------- BVH build + layouts --- */
+
+typedef struct cwbvh_build_params {
+    int32_t num_threads;        /* 0 = all hardware threads */
+    int32_t sah_bins;           /* binned SAH bins per axis (8..256) */
+    float bvh2_node_cost;       /* SAH cost of a BVH2 node visit */
+    float bvh2_tri_cost;        /* SAH cost of a triangle test (BVH2 leaves) */
+    int32_t bvh2_max_leaf;      /* max triangles per BVH2 leaf (1..32) */
+    float cwbvh_node_cost;      /* collapse DP: cost of an 8-wide node */
+    float cwbvh_tri_cost;       /* collapse DP: cost of a triangle */
+    int32_t cwbvh_node_stride;  /* float4 per CWBVH node: 5 (80 B) or 8 (128 B) */
+    int32_t max_depth;          /* BVH2 depth limit (<= CWBVH_BVH2_STACK_CAPACITY-2) */
+} cwbvh_build_params;
+
+int32_t cwbvh_build_params_default(cwbvh_build_params* params);
+
+typedef struct cwbvh_layout cwbvh_layout;
+
+typedef struct cwbvh_layout_view {
+    int32_t kind;                 /* CWBVH_KIND_* */
+    int32_t node_stride;          /* float4 per node */
+    const cwbvh_float4* nodes;
+    int64_t nodes_f4;             /* float4 count of the node array */
+    const cwbvh_float4* woop;
+    int64_t woop_f4;              /* float4 count of the Woop array */
+    const int32_t* prim_idx;      /* one int per Woop float4 (-1 = not a row 0) */
+    int64_t prim_idx_len;
+    int32_t max_depth;            /* deepest node level (root = 1) */
+    int32_t stack_need;           /* traversal stack entries needed */
+    int64_t num_nodes;
+    int64_t num_leaves;
+    int64_t num_tri_refs;
+    double sah_cost;
+} cwbvh_layout_view;
+
+/* Build both layouts from one SAH tree (either output may be NULL). */
+int32_t cwbvh_build_layouts(const cwbvh_mesh* mesh, const cwbvh_build_params* params,
+                            cwbvh_layout** bvh2_out, cwbvh_layout** cwbvh_out);
+/* Wrap caller arrays (e.g. layouts written by another converter). */
+int32_t cwbvh_layout_from_arrays(int32_t kind, int32_t node_stride, const cwbvh_float4* nodes,
+                                 int64_t nodes_f4, const cwbvh_float4* woop, int64_t woop_f4,
+                                 const int32_t* prim_idx, int64_t prim_idx_len,
+                                 cwbvh_layout** out);
+int32_t cwbvh_layout_get_view(const cwbvh_layout* layout, cwbvh_layout_view* view);
+/* Re-lay a CWBVH node array to another node stride (5 <-> 8 float4). */
+int32_t cwbvh_layout_restride(const cwbvh_layout* in, int32_t node_stride, cwbvh_layout** out);
+void cwbvh_layout_free(cwbvh_layout* layout);
+
+/* Woop transform of one triangle (3 float4 rows, -0.0 in row0.x made +0.0;
+ * degenerate triangles get rows that never report a hit). */
+int32_t cwbvh_woopify(const float* v0, const float* v1, const float* v2, cwbvh_float4* out3);
+
+/* ------------------------------------------------- device plumbing --- */
+
+typedef struct cwbvh_device_info {
+    char name[64];
+    char arch[32];
+    int32_t compute_units;
+    int32_t wave_size;
+    int64_t lds_per_block;
+    int64_t l2_bytes;
+    int64_t global_mem_bytes;
+    int32_t clock_khz;
+    int32_t mem_clock_khz;
+    int32_t mem_bus_width;
+} cwbvh_device_info;
+
+int32_t cwbvh_device_count(int32_t* count);
+int32_t cwbvh_get_device_info(int32_t device, cwbvh_device_info* info);
+int32_t cwbvh_malloc(void** dptr, int64_t bytes);
+int32_t cwbvh_free(void* dptr);
+int32_t cwbvh_memcpy_htod(void* dst, const void* src, int64_t bytes);
+int32_t cwbvh_memcpy_dtoh(void* dst, const void* src, int64_t bytes);
+int32_t cwbvh_memcpy_dtod(void* dst, const void* src, int64_t bytes);
+int32_t cwbvh_memset(void* dptr, int32_t value, int64_t bytes);
+int32_t cwbvh_stream_create(void** stream);
+int32_t cwbvh_stream_destroy(void* stream);
+int32_t cwbvh_stream_synchronize(void* stream);
+int32_t cwbvh_device_synchronize(void);
+int32_t cwbvh_event_create(void** event);
+int32_t cwbvh_event_destroy(void* event);
+int32_t cwbvh_event_record(void* event, void* stream);
+int32_t cwbvh_event_synchronize(void* event);
+int32_t cwbvh_event_elapsed_ms(void* start, void* stop, float* ms);
+/* Calibration kernel: dst[i] = src[i] for n uint4 words (HBM copy roofline). */
+int32_t cwbvh_copy_u4(void* dst, const void* src, int64_t n_u4, void* stream);
+
+/* ------------------------------------------------- device scenes --- */
+
+typedef struct cwbvh_scene cwbvh_scene;
+
+/* Upload a host layout (owning device copy). */
+int32_t cwbvh_scene_create(const cwbvh_layout* layout, void* stream, cwbvh_scene** out);
+/* Wrap device arrays without taking ownership (bind semantics). prim_idx may
+ * be NULL; stack_need = traversal stack entries the tree needs (0 = unknown). */
+int32_t cwbvh_scene_wrap_device(int32_t kind, int32_t node_stride, const void* d_nodes,
+                                const void* d_woop, const int32_t* d_prim_idx,
+                                int32_t stack_need, cwbvh_scene** out);
+int32_t cwbvh_scene_device_bytes(const cwbvh_scene* scene, int64_t* bytes);
+int32_t cwbvh_scene_kind(const cwbvh_scene* scene, int32_t* kind);
+void cwbvh_scene_destroy(cwbvh_scene* scene);
+
+/* ------------------------------------------------------------ trace --- */
+
+typedef struct cwbvh_trace_config {
+    int32_t block_size;      /* 64, 128 or 256 threads */
+    int32_t lds_depth;       /* stack entries per lane kept in LDS (0..stack capacity) */
+    int32_t grid_blocks;     /* persistent CWBVH grid (0 = fill the device) */
+    int32_t fetch_chunk;     /* rays claimed per wave-level atomic (CWBVH) */
+    int32_t refill_idle;     /* leave the inner loop once idle lanes exceed this ... */
+    int32_t refill_budget;   /* ... for this many accumulated lane-iterations */
+    int32_t postpone_pct;    /* postpone a triangle group below this % of active lanes (0 = off) */
+    int32_t prefetch;        /* 0 or 1: issue the next node's loads before the triangle loop */
+} cwbvh_trace_config;
+
+int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
+
+/* Traverse n rays.  d_counters (nullable, CWBVH_TRACE_STATS only) receives
+ * uint64[2] = {visited nodes, tested triangles}, accumulated. */
+int32_t cwbvh_trace(const cwbvh_scene* scene, const cwbvh_ray* d_rays, cwbvh_hit* d_hits,
+                    int64_t n, uint32_t flags, const cwbvh_trace_config* cfg,
+                    uint64_t* d_counters, void* stream);
+/* Same, with a ray permutation: ray i of the launch is d_rays[d_order[i]] and
+ * its hit goes to d_hits[d_order[i]]. */
+int32_t cwbvh_trace_ordered(const cwbvh_scene* scene, const cwbvh_ray* d_rays,
+                            const int32_t* d_order, cwbvh_hit* d_hits, int64_t n,
+                            uint32_t flags, const cwbvh_trace_config* cfg,
+                            uint64_t* d_counters, void* stream);
+
+/* 1 when a traversal on this stream overflowed its stack since the last
+ * call (the flag is cleared); the affected rays may have missed hits. */
+int32_t cwbvh_stack_overflowed(void* stream, int32_t* flag);
+
+/* ------------------------------------ reference entry points (drop-in) ---
+ * Same names and argument meaning as the reference's
+ * src/TraversalKernelCWBVH.h:3-11 and src/TraversalKernelBVH2.h:3-11:
+ * bind stores device pointers (non-owning), trace is synchronous on the null
+ * stream, writes one hit per ray and logs "<ms>MS, <rate>MRays/s (...)".
+ * Errors are reported through cwbvh_last_status() instead of aborting. */
+void rtBindCWBVHData(const cwbvh_float4* InBVHTreeNodes,
+                     const cwbvh_float4* InTriangleWoopCoordinates,
+                     const int* InMappingFromTriangleAddressToIndex);
+void rtTraceCWBVH(cwbvh_ray* rayBuffer, cwbvh_hit* rayResultBuffer, int rayCount);
+void rtBindBVH2Data(const cwbvh_float4* InBVHTreeNodes,
+                    const cwbvh_float4* InTriangleWoopCoordinates,
+                    const int* InMappingFromTriangleAddressToIndex);
+void rtTraceBVH2(cwbvh_ray* rayBuffer, cwbvh_hit* rayResultBuffer, int rayCount);
+/* Status and kernel time (ms) of the last rt* call on this host thread. */
+int32_t cwbvh_last_status(void);
+float cwbvh_last_trace_ms(void);
+/* 0 silences the timing line of rtTrace* (default 1). */
+void cwbvh_set_log(int32_t enabled);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CWBVH_AMD_H */
