@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""Benchmark: AO ray traversal Mrays/s on one MI355X (CWBVH vs BVH2).
+
+Default workload is C1: 980,002-triangle icosphere lattice, ~33M AO rays
+(16 per primary hit of a 1920x1080 camera).  One "step" traverses the whole
+AO batch twice with the CWBVH kernel: closest hit, then any hit.  Rays, BVH
+and hit buffers are resident in HBM before the timed region; the timed region
+holds only kernel launches on one stream, bracketed by HIP events.
+
+Prints ONE JSON line.  `value` = CWBVH AO rays traversed per second over the
+step (closest + any), in Mrays/s.  Extra keys report each mode separately, the
+BVH2 kernel on the same rays, the roofline of the dominant kernel (CWBVH
+closest hit) and the CPU baseline (scalar CPU restatement on host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "Mrays/s AO any-hit + closest-hit, CWBVH vs BVH2, 1 GPU; serving p99 ms"
+HBM_PEAK_GBS = 8000.0
+PROFILE_TRAFFIC = ROOT / "profiles" / "cwbvh_closest_c1_traffic.json"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(kind: int, n: int, nodes: int, tris: int, leaves: int = 0) -> int:
+    """Reference-layout bytes: ray 32 + hit 16 per ray, 80 B per CWBVH node or
+    64 B per BVH2 node, 48 B per triangle test, 16 B per BVH2 leaf terminator."""
+    if kind == 1:
+        return 48 * n + 80 * nodes + 48 * tris
+    return 48 * n + 64 * nodes + 48 * tris + 16 * leaves
+
+
+def time_launches(fns, steps: int, warmup: int, cw):
+    """Run fns (list of callables) per step; per-launch times from HIP events."""
+    for _ in range(warmup):
+        for f in fns:
+            f()
+    cw.synchronize()
+    evs = [[cw.Event() for _ in range(len(fns) + 1)] for _ in range(steps)]
+    for s in range(steps):
+        evs[s][0].record()
+        for i, f in enumerate(fns):
+            f()
+            evs[s][i + 1].record()
+    evs[-1][-1].synchronize()
+    per = np.array([[evs[s][i].elapsed_ms(evs[s][i + 1]) for i in range(len(fns))] for s in range(steps)])
+    total_ms = evs[0][0].elapsed_ms(evs[-1][-1])
+    return total_ms, per
+
+
+def cpu_baseline(layout, rays, target_s: float, threads: int) -> dict:
+    import oracle
+
+    nodes, woop = layout.nodes, layout.woop
+    n0 = min(len(rays), 262144)
+    t0 = time.perf_counter()
+    oracle.trace_cwbvh(nodes, layout.node_stride, woop, rays[:n0], threads=threads)
+    dt = time.perf_counter() - t0
+    n = int(min(len(rays), max(n0, n0 * target_s / max(dt, 1e-3))))
+    t0 = time.perf_counter()
+    oracle.trace_cwbvh(nodes, layout.node_stride, woop, rays[:n], threads=threads)
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} AO rays of the workload, CWBVH closest hit, scalar C restatement "
+                      f"(oracle/oracle.c) on {threads} host threads, {dt:.1f} s wall"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--node-stride", type=int, default=5, choices=[5, 8])
+    ap.add_argument("--skip-bvh2", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): rays are independent, so N GPUs "
+              "would each run this same single-GPU workload on 1/N of the rays with a replicated BVH; "
+              "multi-GPU runs are not part of this build.", file=sys.stderr)
+        return 2
+
+    import cwbvh_amd as cw
+    from cwbvh_amd import workloads as W
+
+    if cw.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        return 3
+    dev = cw.device_info(0)
+    log(f"device {dev['arch']} CUs={dev['compute_units']}")
+
+    params = cw.default_build_params(cwbvh_node_stride=args.node_stride)
+    t0 = time.perf_counter()
+    if args.config == "c0":
+        wl = W.c0()
+    else:
+        wl = W.c1(W.gpu_primary_tracer(params))
+    gen_s = time.perf_counter() - t0
+    rays = wl.rays
+    n = len(rays)
+    log(f"workload {wl.name}: {wl.description} ({gen_s:.1f} s)")
+
+    t0 = time.perf_counter()
+    bvh2, cwb = cw.build_layouts(wl.mesh, params)
+    build_s = time.perf_counter() - t0
+    log(f"build {build_s:.2f} s: cwbvh {cwb.stats()} bvh2 {bvh2.stats()}")
+
+    s_cw = cw.Scene(cwb)
+    s_b2 = cw.Scene(bvh2)
+    d_rays = cw.DeviceBuffer.from_numpy(rays)
+    d_hits = cw.DeviceBuffer(16 * n)
+    d_cnt = cw.DeviceBuffer(16)
+    cfg_cw = cw.default_trace_config(cw.KIND_CWBVH)
+    cfg_b2 = cw.default_trace_config(cw.KIND_BVH2)
+
+    # algorithmic traffic (counting pass, outside the timed region)
+    stats = {}
+    for name, sc, cfg, fl in (("cwbvh_closest", s_cw, cfg_cw, 0), ("cwbvh_any", s_cw, cfg_cw, 1),
+                              ("bvh2_closest", s_b2, cfg_b2, 0), ("bvh2_any", s_b2, cfg_b2, 1)):
+        d_cnt.zero()
+        sc.trace(d_rays.ptr, d_hits.ptr, n, fl | cw.TRACE_STATS, cfg, d_cnt.ptr)
+        cw.synchronize()
+        c = d_cnt.download((2,), np.uint64)
+        stats[name] = (int(c[0]), int(c[1]))
+
+    def run(sc, fl, cfg):
+        return lambda: sc.trace(d_rays.ptr, d_hits.ptr, n, fl, cfg)
+
+    log("timing CWBVH closest + any")
+    total_ms, per = time_launches([run(s_cw, 0, cfg_cw), run(s_cw, 1, cfg_cw)], args.steps, args.warmup, cw)
+    ms_step = total_ms / args.steps
+    ms_closest = float(per[:, 0].mean())
+    ms_any = float(per[:, 1].mean())
+    modes = {"cwbvh_closest": n / ms_closest / 1e3, "cwbvh_any": n / ms_any / 1e3}
+    if not args.skip_bvh2:
+        log("timing BVH2 closest + any")
+        _, per2 = time_launches([run(s_b2, 0, cfg_b2), run(s_b2, 1, cfg_b2)], args.steps, args.warmup, cw)
+        modes["bvh2_closest"] = n / float(per2[:, 0].mean()) / 1e3
+        modes["bvh2_any"] = n / float(per2[:, 1].mean()) / 1e3
+    overflow = cw.stack_overflowed()
+
+    nodes_c, tris_c = stats["cwbvh_closest"]
+    bytes_c = algorithmic_bytes(1, n, nodes_c, tris_c)
+    achieved = bytes_c / (ms_closest / 1e3) / 1e9
+    traffic = None
+    if PROFILE_TRAFFIC.exists():
+        try:
+            t = json.loads(PROFILE_TRAFFIC.read_text())
+            if t.get("workload") == wl.name and t.get("rays") == n:
+                traffic = t.get("hbm_bytes_per_launch")
+        except (ValueError, OSError):
+            traffic = None
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "cwbvh_trace<closest> (persistent CWBVH, closest hit)",
+            "algorithmic_bytes_per_launch": bytes_c, "avg_launch_ms": round(ms_closest, 4),
+            "per_ray": {"nodes": nodes_c / n, "triangles": tris_c / n, "bytes": bytes_c / n}}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        log(f"cpu baseline on {threads} threads")
+        cpu = cpu_baseline(cwb, rays, args.cpu_seconds, threads)
+
+    value = 2 * n / (ms_step / 1e3) / 1e6
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{wl.name}: {wl.description}", "triangles": wl.mesh.num_triangles,
+                   "rays_per_step": 2 * n, "ao_rays": n, "seed": wl.seed,
+                   "step": "CWBVH closest-hit pass + CWBVH any-hit pass over the AO batch",
+                   "node_stride_f4": args.node_stride},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "modes_mrays_s": {k: round(v, 2) for k, v in modes.items()},
+        "cwbvh_vs_bvh2": {k: round(modes[f"cwbvh_{k}"] / modes[f"bvh2_{k}"], 3) for k in ("closest", "any")
+                          if f"bvh2_{k}" in modes},
+        "traversal_counts": {k: {"nodes_per_ray": v[0] / n, "tris_per_ray": v[1] / n} for k, v in stats.items()},
+        "trace_config": {"cwbvh": cfg_cw.as_dict(), "bvh2": cfg_b2.as_dict()},
+        "build_s": round(build_s, 2),
+        "stack_overflow": overflow,
+        "device": dev["arch"],
+    }
+    if wl.extra.get("primary_hit_rate") is not None:
+        out["config"]["primary_hit_rate"] = round(wl.extra["primary_hit_rate"], 4)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

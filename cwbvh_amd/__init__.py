@@ -83,6 +83,7 @@ class TraceConfig(C.Structure):
         ("refill_budget", C.c_int32),
         ("postpone_pct", C.c_int32),
         ("prefetch", C.c_int32),
+        ("work_slices", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -122,7 +123,7 @@ _SIGS = {
     "cwbvh_mesh_bounds": (_I32, [_P, _P, _P]),
     "cwbvh_mesh_free": (None, [_P]),
     "cwbvh_mesh_heightfield": (_I32, [_I32, _I32, _I32, _F, _U64, _PP]),
-    "cwbvh_mesh_icosphere_grid": (_I32, [_I32, _I32, _I32, _I32, _F, _F, _F, _I32, _U64, _PP]),
+    "cwbvh_mesh_icosphere_grid": (_I32, [_I32, _I32, _I32, _I32, _F, _F, _F, _F, _U64, _PP]),
     "cwbvh_mesh_cluster_soup": (_I32, [_I64, _I32, _F, _F, _F, _F, _U64, _PP]),
     "cwbvh_mesh_validation_planes": (_I32, [_I32, _PP]),
     "cwbvh_rays_ao_surface": (_I32, [_P, _I64, _I32, _F, _F, _F, _U64, _P]),
@@ -241,9 +242,10 @@ class Mesh:
 
     @classmethod
     def icosphere_grid(cls, gx=10, gy=10, gz=10, frequency=7, spacing=4.0, scale_min=0.5, scale_max=1.5,
-                       ground_plane=True, seed=0) -> "Mesh":
+                       ground_margin=0.5, seed=0) -> "Mesh":
+        """ground_margin < 0 leaves out the ground square."""
         return cls._make(lib.cwbvh_mesh_icosphere_grid, gx, gy, gz, frequency, spacing, scale_min, scale_max,
-                         1 if ground_plane else 0, seed)
+                         float(ground_margin), seed)
 
     @classmethod
     def cluster_soup(cls, num_triangles, num_clusters=4096, sigma_min=0.005, sigma_max=0.05,

@@ -52,8 +52,8 @@ int32_t stream_counters(hipStream_t stream, unsigned** out) {
         return CWBVH_OK;
     }
     unsigned* p = nullptr;
-    CW_HIP(hipMalloc(&p, 4 * sizeof(unsigned)));
-    CW_HIP(hipMemset(p, 0, 4 * sizeof(unsigned)));
+    CW_HIP(hipMalloc(&p, kWorkWords * sizeof(unsigned)));
+    CW_HIP(hipMemset(p, 0, kWorkWords * sizeof(unsigned)));
     table[stream] = p;
     *out = p;
     return CWBVH_OK;
@@ -102,8 +102,8 @@ int32_t cwbvh_get_device_info(int32_t device, cwbvh_device_info* info) {
     hipDeviceProp_t p;
     CW_HIP(hipGetDeviceProperties(&p, device));
     std::memset(info, 0, sizeof(*info));
-    std::snprintf(info->name, sizeof(info->name), "%s", p.name);
-    std::snprintf(info->arch, sizeof(info->arch), "%s", p.gcnArchName);
+    std::snprintf(info->name, sizeof(info->name), "%.63s", p.name);
+    std::snprintf(info->arch, sizeof(info->arch), "%.31s", p.gcnArchName);
     info->compute_units = p.multiProcessorCount;
     info->wave_size = p.warpSize;
     info->lds_per_block = (int64_t)p.sharedMemPerBlock;
@@ -325,9 +325,11 @@ int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* c) {
         c->refill_budget = 32;
         c->postpone_pct = 0;
         c->prefetch = 0;
+        c->work_slices = 8;
     } else {
         c->block_size = 128;
         c->lds_depth = 16;
+        c->work_slices = 1;
     }
     return CWBVH_OK;
 }
@@ -380,10 +382,10 @@ int32_t cwbvh_stack_overflowed(void* stream, int32_t* flag) {
     int32_t rc = stream_counters((hipStream_t)stream, &c);
     if (rc) return rc;
     unsigned v = 0;
-    CW_HIP(hipMemcpyAsync(&v, c + 2, sizeof(unsigned), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    CW_HIP(hipMemcpyAsync(&v, c + kWorkOverflow, sizeof(unsigned), hipMemcpyDeviceToHost, (hipStream_t)stream));
     CW_HIP(hipStreamSynchronize((hipStream_t)stream));
     *flag = v ? 1 : 0;
-    if (v) CW_HIP(hipMemsetAsync(c + 2, 0, sizeof(unsigned), (hipStream_t)stream));
+    if (v) CW_HIP(hipMemsetAsync(c + kWorkOverflow, 0, sizeof(unsigned), (hipStream_t)stream));
     return CWBVH_OK;
 }
 

@@ -71,27 +71,53 @@ __device__ __forceinline__ bool woop_test(const float4 v0, const float4 v1, cons
 // Traversal stack: the first `depth` entries of every lane live in LDS,
 // interleaved by lane (entry k of thread tid at k * blockDim + tid, so a wave
 // touches consecutive words); deeper entries spill to a per-lane scratch array.
-template <class T, int CAP>
-struct Stack {
-    T* lds;        // this thread's column base: lds_base + tid
-    int stride;    // blockDim.x
-    int depth;     // entries held in LDS
-    T spill[CAP];
-
-    __device__ __forceinline__ void init(T* base, int tid, int block, int d) {
-        lds = base + tid;
-        stride = block;
-        depth = d;
-    }
-    __device__ __forceinline__ void put(int sp, T v) {
-        if (sp < depth) lds[sp * stride] = v;
-        else spill[sp - depth] = v;
-    }
-    __device__ __forceinline__ T get(int sp) const {
-        if (sp < depth) return lds[sp * stride];
-        return spill[sp - depth];
-    }
+// The LDS array and the spill array are passed separately so that the index
+// state stays in registers and LDS is addressed with ds_* instructions.
+struct StackIndex {
+    int col;     // threadIdx.x
+    int stride;  // blockDim.x
+    int depth;   // entries held in LDS
 };
+
+#define CWD_LDS __attribute__((address_space(3)))
+
+template <class T>
+__device__ __forceinline__ CWD_LDS T* lds_ptr(T* p) {
+    return (CWD_LDS T*)p;
+}
+
+template <class T, int CAP>
+__device__ __forceinline__ void stack_put(CWD_LDS T* lds, T (&spill)[CAP], const StackIndex& s, int sp, T v) {
+    if (sp < s.depth) lds[sp * s.stride + s.col] = v;
+    else spill[sp - s.depth] = v;
+}
+
+// The LDS pointer keeps its address space in the type, so the two loads below
+// cannot be merged into one generic (flat) load through a selected pointer.
+template <class T, int CAP>
+__device__ __forceinline__ T stack_get(const CWD_LDS T* lds, const T (&spill)[CAP], const StackIndex& s, int sp) {
+    if (sp < s.depth) return lds[sp * s.stride + s.col];
+    return spill[sp - s.depth];
+}
+
+// Streaming (read-once / write-once) 16-byte accesses for rays and hits.
+typedef float f4v __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 nt_load4(const float4* p) {
+    const f4v v = __builtin_nontemporal_load(reinterpret_cast<const f4v*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void nt_store4(float4 v, float4* p) {
+    const f4v w = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(w, reinterpret_cast<f4v*>(p));
+}
+
+// CWBVH stack entries (node or triangle group) travel as one 64-bit word.
+__device__ __forceinline__ unsigned long long pack2(uint2 v) {
+    return ((unsigned long long)v.y << 32) | (unsigned long long)v.x;
+}
+__device__ __forceinline__ uint2 unpack2(unsigned long long v) {
+    return make_uint2((unsigned)(v & 0xFFFFFFFFull), (unsigned)(v >> 32));
+}
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 

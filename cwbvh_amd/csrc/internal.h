@@ -32,10 +32,15 @@ int32_t hip_fail(hipError_t e, const char* what);
         if (e_ != hipSuccess) return ::cw::hip_fail(e_, #call); \
     } while (0)
 
-// Per-stream work counters for persistent launches: uint32[4] =
-// {ray chunk counter, exited waves, stack overflow flag, spare}.  The kernel
-// resets [0] and [1] when its last wave exits, so launches on one stream can
-// reuse the same counters (and a captured graph replays cleanly).
+// Per-stream work counters for persistent launches: uint32[kWorkWords] =
+// {fetch counters [0, kWorkSlices), exited waves, stack overflow flag, ...}.
+// The kernel resets the fetch counters and the exit count when its last wave
+// exits, so launches on one stream reuse the same counters (and a captured
+// graph replays cleanly).  The overflow flag stays set until read.
+constexpr int kWorkSlices = 8;
+constexpr int kWorkExited = 8;
+constexpr int kWorkOverflow = 9;
+constexpr int kWorkWords = 16;
 int32_t stream_counters(hipStream_t stream, unsigned** out);
 
 struct TraceLaunch {

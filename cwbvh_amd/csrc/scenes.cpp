@@ -247,7 +247,7 @@ int32_t cwbvh_mesh_heightfield(int32_t nx, int32_t ny, int32_t octaves, float am
 }
 
 int32_t cwbvh_mesh_icosphere_grid(int32_t gx, int32_t gy, int32_t gz, int32_t frequency, float spacing,
-                                  float scale_min, float scale_max, int32_t ground_plane, uint64_t seed,
+                                  float scale_min, float scale_max, float ground_margin, uint64_t seed,
                                   cwbvh_mesh** out) {
     CW_CHECK_ARG(out != nullptr, "out is NULL");
     CW_CHECK_ARG(gx >= 1 && gy >= 1 && gz >= 1 && frequency >= 1 && frequency <= 256, "bad grid");
@@ -282,10 +282,13 @@ int32_t cwbvh_mesh_icosphere_grid(int32_t gx, int32_t gy, int32_t gz, int32_t fr
                     m->indices.push_back(base + t[2]);
                 }
             }
-    if (ground_plane) {
+    if (ground_margin >= 0.f) {
+        // square below the lattice, extending ground_margin lattice extents
+        // beyond it on every side
         float z = -0.5f * spacing - scale_max;
-        float x0 = -2.0f * spacing, y0 = -2.0f * spacing;
-        float x1 = (gx + 1) * spacing, y1 = (gy + 1) * spacing;
+        float ex = (gx - 1) * spacing, ey = (gy - 1) * spacing;
+        float x0 = -ground_margin * ex - spacing, y0 = -ground_margin * ey - spacing;
+        float x1 = (1.0f + ground_margin) * ex + spacing, y1 = (1.0f + ground_margin) * ey + spacing;
         int32_t base = (int32_t)(m->vertices.size() / 3);
         float q[12] = {x0, y0, z, x1, y0, z, x1, y1, z, x0, y1, z};
         m->vertices.insert(m->vertices.end(), q, q + 12);

@@ -38,6 +38,9 @@ using namespace cwd;
 namespace {
 
 constexpr int kMaxBlock = 256;
+constexpr int kWorkSlices = cw::kWorkSlices;
+constexpr int kWorkExited = cw::kWorkExited;
+constexpr int kWorkOverflow = cw::kWorkOverflow;
 
 struct CwbvhParams {
     const float4* __restrict__ nodes;
@@ -47,9 +50,10 @@ struct CwbvhParams {
     const int* __restrict__ order;
     float4* __restrict__ hits;
     const cw::BatchJob* __restrict__ jobs;
-    unsigned* work;                 // [0] fetch counter, [1] exited waves, [2] overflow
+    unsigned* work;                 // fetch counters, exited waves, overflow (kWork*)
     unsigned long long* stats;      // {nodes, triangles}
     int n;                          // rays (single) or total chunks (multi)
+    int slices;                     // single scene: contiguous ray ranges, one per XCD
     int num_jobs;
     int stride;                     // float4 per node
     int lds_depth;
@@ -90,11 +94,12 @@ __device__ __forceinline__ unsigned child_hits4(unsigned meta4, unsigned lox, un
     return hitmask;
 }
 
-template <bool ANY, bool STATS, bool MULTI>
+template <bool ANY, bool STATS, bool MULTI, bool PREFETCH>
 __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
-    extern __shared__ uint2 lds_stack[];
-    Stack<uint2, kStackCapacity> stack;
-    stack.init(lds_stack, threadIdx.x, blockDim.x, P.lds_depth);
+    extern __shared__ unsigned long long lds_stack_raw[];
+    CWD_LDS unsigned long long* lds_stack = lds_ptr(lds_stack_raw);
+    unsigned long long spill[kStackCapacity];
+    const StackIndex sx{(int)threadIdx.x, (int)blockDim.x, P.lds_depth};
     const int lane = lane_id();
 
     // per-lane ray state
@@ -112,14 +117,25 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
     int hit_addr = -1;
     float hu = 0.f, hv = 0.f;
     unsigned long long st_nodes = 0, st_tris = 0;
+    bool have_pf = false;
+    float4 pf0, pf1, pf2, pf3, pf4;
+    // A lane is busy from loading a ray until writing its hit.  (Deriving
+    // "terminated" from an empty stack and empty groups, as the reference
+    // does at src/TraversalKernelCWBVH.cu:60, misreads a lane whose last
+    // stack entry was a postponed triangle group just popped into ng.)
+    bool busy = false;
 
-    // wave-uniform work pool
+    // wave-uniform work pool; single-scene launches split the rays into
+    // `slices` contiguous ranges and a wave drains the range of its XCD
+    // (workgroups are dispatched round-robin over the XCDs) before stealing
     unsigned pool_next = 0, pool_end = 0;
     int pool_job = 0;
     bool exhausted = false;
+    const int home = MULTI ? 0 : (int)(blockIdx.x % (unsigned)P.slices);
+    int slice_k = 0;
 
     while (true) {
-        const bool terminated = sp == 0 && ng.y <= 0x00FFFFFFu && tg.y == 0u;
+        const bool terminated = !busy;
         unsigned long long need = __ballot(terminated);
         bool got = false;
         unsigned my = 0;
@@ -128,14 +144,14 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
             if (pool_next >= pool_end) {
                 if (exhausted) break;
                 const int leader = __ffsll((long long)need) - 1;
-                unsigned base = 0;
-                if (lane == leader) base = atomicAdd(&P.work[0], MULTI ? 1u : (unsigned)P.chunk);
-                base = __shfl(base, leader);
-                if (base >= (unsigned)P.n) {
-                    exhausted = true;
-                    break;
-                }
                 if (MULTI) {
+                    unsigned base = 0;
+                    if (lane == leader) base = atomicAdd(&P.work[0], 1u);
+                    base = __shfl(base, leader);
+                    if (base >= (unsigned)P.n) {
+                        exhausted = true;
+                        break;
+                    }
                     // job j owns chunks [jobs[j].chunk_begin, jobs[j+1].chunk_begin)
                     int lo = 0, hi = P.num_jobs - 1;
                     while (lo < hi) {
@@ -148,8 +164,26 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
                     pool_next = first;
                     pool_end = min(first + (unsigned)P.chunk, (unsigned)P.jobs[lo].n);
                 } else {
-                    pool_next = base;
-                    pool_end = min(base + (unsigned)P.chunk, (unsigned)P.n);
+                    bool found = false;
+                    while (slice_k < P.slices) {
+                        const int sl = (home + slice_k) % P.slices;
+                        const unsigned lo = (unsigned)(((unsigned long long)P.n * (unsigned)sl) / (unsigned)P.slices);
+                        const unsigned hi = (unsigned)(((unsigned long long)P.n * (unsigned)(sl + 1)) / (unsigned)P.slices);
+                        unsigned base = 0;
+                        if (lane == leader) base = atomicAdd(&P.work[sl], (unsigned)P.chunk);
+                        base = __shfl(base, leader);
+                        if (base < hi - lo) {
+                            pool_next = lo + base;
+                            pool_end = lo + min(base + (unsigned)P.chunk, hi - lo);
+                            found = true;
+                            break;
+                        }
+                        ++slice_k;
+                    }
+                    if (!found) {
+                        exhausted = true;
+                        break;
+                    }
                 }
             }
             const unsigned avail = pool_end - pool_next;
@@ -179,7 +213,9 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
             }
             const int ri = order ? order[my] : (int)my;
             out_idx = ri;
-            const float4 ro = rays[ri].o, rd = rays[ri].d;
+            // rays are read once: keep them out of the caches the BVH lives in
+            const float4 ro = nt_load4(&rays[ri].o);
+            const float4 rd = nt_load4(&rays[ri].d);
             o = make_float3(ro.x, ro.y, ro.z);
             d = make_float3(rd.x, rd.y, rd.z);
             tmin = ro.w;
@@ -194,6 +230,8 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
             sp = 0;
             hit_addr = -1;
             hu = hv = 0.f;
+            have_pf = false;
+            busy = true;
         }
 
         int lost = 0;
@@ -205,16 +243,22 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
                 ng.y &= ~(1u << bit);
                 if (ng.y > 0x00FFFFFFu) {
                     if (sp < kStackCapacity) {
-                        stack.put(sp, ng);
+                        stack_put(lds_stack, spill, sx, sp, pack2(ng));
                         ++sp;
                     } else {
-                        P.work[2] = 1u;
+                        P.work[kWorkOverflow] = 1u;
                     }
                 }
-                const unsigned slot = (unsigned)(bit - 24) ^ octinv;
-                const unsigned rel = (unsigned)__popc(imask & ~(0xFFFFFFFFu << slot));
-                const float4* np = nodes + (size_t)(base + rel) * (unsigned)stride;
-                const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
+                float4 n0, n1, n2, n3, n4;
+                if (PREFETCH && have_pf) {
+                    n0 = pf0; n1 = pf1; n2 = pf2; n3 = pf3; n4 = pf4;
+                    have_pf = false;
+                } else {
+                    const unsigned slot = (unsigned)(bit - 24) ^ octinv;
+                    const unsigned rel = (unsigned)__popc(imask & ~(0xFFFFFFFFu << slot));
+                    const float4* np = nodes + (size_t)(base + rel) * (unsigned)stride;
+                    n0 = np[0]; n1 = np[1]; n2 = np[2]; n3 = np[3]; n4 = np[4];
+                }
                 if (STATS) ++st_nodes;
                 const unsigned eb = f2u(n0.w);
                 const int ex = (int)(signed char)(eb & 0xFFu);
@@ -227,26 +271,36 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
                 const float origy = -(o.y - n0.y) * idy;
                 const float origz = -(o.z - n0.z) * idz;
                 const unsigned octinv4 = octinv * 0x01010101u;
-                const bool sx = idx < 0.f, sy = idy < 0.f, sz = idz < 0.f;
+                const bool sgx = idx < 0.f, sgy = idy < 0.f, sgz = idz < 0.f;
                 unsigned hitmask;
                 {
                     const unsigned qlx = f2u(n2.x), qly = f2u(n2.z), qlz = f2u(n3.x);
                     const unsigned qhx = f2u(n3.z), qhy = f2u(n4.x), qhz = f2u(n4.z);
-                    hitmask = child_hits4(f2u(n1.z), sx ? qhx : qlx, sy ? qhy : qly, sz ? qhz : qlz,
-                                          sx ? qlx : qhx, sy ? qly : qhy, sz ? qlz : qhz, adjx, adjy, adjz,
+                    hitmask = child_hits4(f2u(n1.z), sgx ? qhx : qlx, sgy ? qhy : qly, sgz ? qhz : qlz,
+                                          sgx ? qlx : qhx, sgy ? qly : qhy, sgz ? qlz : qhz, adjx, adjy, adjz,
                                           origx, origy, origz, tmin, tmax, octinv4);
                 }
                 {
                     const unsigned qlx = f2u(n2.y), qly = f2u(n2.w), qlz = f2u(n3.y);
                     const unsigned qhx = f2u(n3.w), qhy = f2u(n4.y), qhz = f2u(n4.w);
-                    hitmask |= child_hits4(f2u(n1.w), sx ? qhx : qlx, sy ? qhy : qly, sz ? qhz : qlz,
-                                           sx ? qlx : qhx, sy ? qly : qhy, sz ? qlz : qhz, adjx, adjy, adjz,
+                    hitmask |= child_hits4(f2u(n1.w), sgx ? qhx : qlx, sgy ? qhy : qly, sgz ? qhz : qlz,
+                                           sgx ? qlx : qhx, sgy ? qly : qhy, sgz ? qlz : qhz, adjx, adjy, adjz,
                                            origx, origy, origz, tmin, tmax, octinv4);
                 }
                 ng.x = f2u(n1.x);
                 ng.y = (hitmask & 0xFF000000u) | (eb >> 24);
                 tg.x = f2u(n1.y);
                 tg.y = hitmask & 0x00FFFFFFu;
+                if (PREFETCH && ng.y > 0x00FFFFFFu) {
+                    // the next iteration visits the highest hit child: start its
+                    // loads now so they overlap this node's triangle tests
+                    const int nb = 31 - __clz((int)ng.y);
+                    const unsigned nslot = (unsigned)(nb - 24) ^ octinv;
+                    const unsigned nrel = (unsigned)__popc(ng.y & ~(0xFFFFFFFFu << nslot));
+                    const float4* np = nodes + (size_t)(ng.x + nrel) * (unsigned)stride;
+                    pf0 = np[0]; pf1 = np[1]; pf2 = np[2]; pf3 = np[3]; pf4 = np[4];
+                    have_pf = true;
+                }
             } else {
                 tg = ng;
                 ng = make_uint2(0u, 0u);
@@ -259,10 +313,10 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
                     const int active = __popcll(__ballot(1));
                     if (active * 100 < total * P.postpone_pct) {
                         if (sp < kStackCapacity) {
-                            stack.put(sp, tg);
+                            stack_put(lds_stack, spill, sx, sp, pack2(tg));
                             ++sp;
                         } else {
-                            P.work[2] = 1u;
+                            P.work[kWorkOverflow] = 1u;
                         }
                         tg.y = 0u;
                         break;
@@ -289,17 +343,19 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
             if (!finished && ng.y <= 0x00FFFFFFu) {
                 if (sp > 0) {
                     --sp;
-                    ng = stack.get(sp);
+                    ng = unpack2(stack_get(lds_stack, spill, sx, sp));
                 } else {
                     finished = true;
                 }
             }
             if (finished) {
                 const int out_id = (hit_addr >= 0 && P.prim_id) ? prim_map[hit_addr] : hit_addr;
-                hits[out_idx] = make_float4(tmax, i2f(out_id), hu, hv);
+                nt_store4(make_float4(tmax, i2f(out_id), hu, hv), &hits[out_idx]);
                 sp = 0;
                 ng = make_uint2(0u, 0u);
                 tg = make_uint2(0u, 0u);
+                have_pf = false;
+                busy = false;
                 break;
             }
             lost += __popcll(~__ballot(1)) - P.idle_lanes;
@@ -315,10 +371,10 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
     __threadfence();
     const unsigned long long alive = __ballot(1);
     if (lane == __ffsll((long long)alive) - 1) {
-        const unsigned prev = atomicAdd(&P.work[1], 1u);
+        const unsigned prev = atomicAdd(&P.work[kWorkExited], 1u);
         if (prev == (unsigned)P.total_waves - 1u) {
-            atomicExch(&P.work[0], 0u);
-            atomicExch(&P.work[1], 0u);
+            for (int k = 0; k < kWorkSlices; ++k) atomicExch(&P.work[k], 0u);
+            atomicExch(&P.work[kWorkExited], 0u);
         }
     }
 }
@@ -354,14 +410,16 @@ __device__ __forceinline__ bool box2(float lox, float hix, float loy, float hiy,
 template <bool ANY, bool STATS>
 __global__ __launch_bounds__(kMaxBlock) void bvh2_trace(Bvh2Params P) {
     constexpr int SENTINEL = 0x76543210;
-    extern __shared__ int lds_stack_i[];
+    extern __shared__ int lds_stack_raw_i[];
+    CWD_LDS int* lds_stack_i = lds_ptr(lds_stack_raw_i);
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
-    Stack<int, kBvh2StackCapacity> stack;
-    stack.init(lds_stack_i, threadIdx.x, blockDim.x, P.lds_depth);
+    int spill[kBvh2StackCapacity];
+    const StackIndex sx{(int)threadIdx.x, (int)blockDim.x, P.lds_depth};
 
     const int ri = P.order ? P.order[i] : i;
-    const float4 ro = P.rays[ri].o, rd = P.rays[ri].d;
+    const float4 ro = nt_load4(&P.rays[ri].o);
+    const float4 rd = nt_load4(&P.rays[ri].d);
     const float3 o = make_float3(ro.x, ro.y, ro.z), d = make_float3(rd.x, rd.y, rd.z);
     const float tmin = ro.w;
     float hitT = rd.w;
@@ -370,7 +428,7 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace(Bvh2Params P) {
     unsigned long long st_nodes = 0, st_tris = 0;
 
     int sp = 0;
-    stack.put(0, SENTINEL);
+    stack_put(lds_stack_i, spill, sx, 0, SENTINEL);
     int node = 0, leaf = 0, hit_addr = -1;
     float hu = 0.f, hv = 0.f;
     bool done = false;
@@ -385,7 +443,7 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace(Bvh2Params P) {
             const bool t1 = box2(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, idx, idy, idz, oodx, oody, oodz, tmin, hitT, d1);
             const int c0 = f2i(n3.x), c1 = f2i(n3.y);
             if (!t0 && !t1) {
-                node = stack.get(sp);
+                node = stack_get(lds_stack_i, spill, sx, sp);
                 --sp;
             } else {
                 node = t0 ? c0 : c1;
@@ -397,15 +455,15 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace(Bvh2Params P) {
                     }
                     if (sp + 1 < kBvh2StackCapacity) {
                         ++sp;
-                        stack.put(sp, far);
+                        stack_put(lds_stack_i, spill, sx, sp, far);
                     } else {
-                        P.work[2] = 1u;
+                        P.work[kWorkOverflow] = 1u;
                     }
                 }
             }
             if (node < 0 && leaf >= 0) {
                 leaf = node;
-                node = stack.get(sp);
+                node = stack_get(lds_stack_i, spill, sx, sp);
                 --sp;
             }
             if (!__any(leaf >= 0)) break;
@@ -432,14 +490,14 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace(Bvh2Params P) {
             if (done) break;
             leaf = node;
             if (node < 0) {
-                node = stack.get(sp);
+                node = stack_get(lds_stack_i, spill, sx, sp);
                 --sp;
             }
         }
         if (done) break;
     }
     const int out_id = (hit_addr >= 0 && P.prim_id) ? P.prim_map[hit_addr] : hit_addr;
-    P.hits[ri] = make_float4(hitT, i2f(out_id), hu, hv);
+    nt_store4(make_float4(hitT, i2f(out_id), hu, hv), &P.hits[ri]);
     if (STATS) {
         atomicAdd(&P.stats[0], st_nodes);
         atomicAdd(&P.stats[1], st_tris);
@@ -468,11 +526,11 @@ int resident_blocks(K kernel, int block, size_t lds) {
     return nb;
 }
 
-template <bool ANY, bool STATS, bool MULTI>
+template <bool ANY, bool STATS, bool MULTI, bool PREFETCH>
 int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t stream) {
     const int block = cfg.block_size;
-    const size_t lds = (size_t)block * (size_t)cfg.lds_depth * sizeof(uint2);
-    auto kernel = cwbvh_trace<ANY, STATS, MULTI>;
+    const size_t lds = (size_t)block * (size_t)cfg.lds_depth * sizeof(unsigned long long);
+    auto kernel = cwbvh_trace<ANY, STATS, MULTI, PREFETCH>;
     int grid = cfg.grid_blocks;
     if (grid <= 0) grid = resident_blocks(kernel, block, lds) * cw::device_compute_units();
     p.total_waves = grid * (block / 64);
@@ -489,6 +547,22 @@ int32_t launch_bvh2_t(Bvh2Params& p, const cwbvh_trace_config& cfg, hipStream_t 
     hipLaunchKernelGGL((bvh2_trace<ANY, STATS>), dim3((unsigned)grid), dim3(block), lds, stream, p);
     CW_HIP(hipGetLastError());
     return CWBVH_OK;
+}
+
+template <bool MULTI>
+int32_t dispatch_cwbvh(CwbvhParams& p, uint32_t flags, const cwbvh_trace_config& cfg, hipStream_t st) {
+    const bool any = (flags & CWBVH_TRACE_ANY) != 0, stats = (flags & CWBVH_TRACE_STATS) != 0;
+    const bool pf = cfg.prefetch != 0;
+    if (any) {
+        if (stats) return pf ? launch_cwbvh_t<true, true, MULTI, true>(p, cfg, st)
+                             : launch_cwbvh_t<true, true, MULTI, false>(p, cfg, st);
+        return pf ? launch_cwbvh_t<true, false, MULTI, true>(p, cfg, st)
+                  : launch_cwbvh_t<true, false, MULTI, false>(p, cfg, st);
+    }
+    if (stats) return pf ? launch_cwbvh_t<false, true, MULTI, true>(p, cfg, st)
+                         : launch_cwbvh_t<false, true, MULTI, false>(p, cfg, st);
+    return pf ? launch_cwbvh_t<false, false, MULTI, true>(p, cfg, st)
+              : launch_cwbvh_t<false, false, MULTI, false>(p, cfg, st);
 }
 
 }  // namespace
@@ -513,7 +587,7 @@ int32_t validate_config(int32_t kind, const cwbvh_trace_config& c) {
     if (kind == CWBVH_KIND_CWBVH) {
         if (c.fetch_chunk < 1 || c.fetch_chunk > (1 << 20) || c.grid_blocks < 0 || c.refill_budget < 1 ||
             c.refill_idle < 0 || c.refill_idle > 64 || c.postpone_pct < 0 || c.postpone_pct > 100 ||
-            c.prefetch < 0 || c.prefetch > 1) {
+            c.prefetch < 0 || c.prefetch > 1 || c.work_slices < 1 || c.work_slices > kWorkSlices) {
             set_error("invalid CWBVH trace configuration");
             return CWBVH_ERR_INVALID_ARGUMENT;
         }
@@ -542,11 +616,8 @@ int32_t launch_cwbvh(const TraceLaunch& L) {
     p.idle_budget = L.cfg.refill_budget;
     p.postpone_pct = L.cfg.postpone_pct;
     p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
-    const bool any = (L.flags & CWBVH_TRACE_ANY) != 0, stats = (L.flags & CWBVH_TRACE_STATS) != 0;
-    if (any) return stats ? launch_cwbvh_t<true, true, false>(p, L.cfg, L.stream)
-                          : launch_cwbvh_t<true, false, false>(p, L.cfg, L.stream);
-    return stats ? launch_cwbvh_t<false, true, false>(p, L.cfg, L.stream)
-                 : launch_cwbvh_t<false, false, false>(p, L.cfg, L.stream);
+    p.slices = L.cfg.work_slices;
+    return dispatch_cwbvh<false>(p, L.flags, L.cfg, L.stream);
 }
 
 int32_t launch_cwbvh_batch(const BatchLaunch& L) {
@@ -562,11 +633,8 @@ int32_t launch_cwbvh_batch(const BatchLaunch& L) {
     p.idle_budget = L.cfg.refill_budget;
     p.postpone_pct = L.cfg.postpone_pct;
     p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
-    const bool any = (L.flags & CWBVH_TRACE_ANY) != 0, stats = (L.flags & CWBVH_TRACE_STATS) != 0;
-    if (any) return stats ? launch_cwbvh_t<true, true, true>(p, L.cfg, L.stream)
-                          : launch_cwbvh_t<true, false, true>(p, L.cfg, L.stream);
-    return stats ? launch_cwbvh_t<false, true, true>(p, L.cfg, L.stream)
-                 : launch_cwbvh_t<false, false, true>(p, L.cfg, L.stream);
+    p.slices = 1;
+    return dispatch_cwbvh<true>(p, L.flags, L.cfg, L.stream);
 }
 
 int32_t launch_bvh2(const TraceLaunch& L) {

@@ -104,11 +104,12 @@ void cwbvh_mesh_free(cwbvh_mesh* mesh);
 int32_t cwbvh_mesh_heightfield(int32_t nx, int32_t ny, int32_t octaves, float amp0,
                                uint64_t seed, cwbvh_mesh** out);
 /* gx*gy*gz randomly rotated geodesic spheres (20*freq^2 triangles each) with
- * radius U[scale_min, scale_max] on a lattice of pitch `spacing`, plus an
- * optional 2-triangle ground plane below the lattice. */
+ * radius U[scale_min, scale_max] on a lattice of pitch `spacing`, plus a
+ * 2-triangle ground square below the lattice reaching ground_margin lattice
+ * extents (+ one pitch) beyond it on every side (ground_margin < 0: none). */
 int32_t cwbvh_mesh_icosphere_grid(int32_t gx, int32_t gy, int32_t gz, int32_t frequency,
                                   float spacing, float scale_min, float scale_max,
-                                  int32_t ground_plane, uint64_t seed, cwbvh_mesh** out);
+                                  float ground_margin, uint64_t seed, cwbvh_mesh** out);
 /* Triangle soup in Gaussian clusters inside the unit cube: cluster sigma
  * log-uniform in [sigma_min, sigma_max], edge lengths lognormal(mu, s). */
 int32_t cwbvh_mesh_cluster_soup(int64_t num_triangles, int32_t num_clusters,
@@ -262,6 +263,7 @@ typedef struct cwbvh_trace_config {
     int32_t refill_budget;   /* ... for this many accumulated lane-iterations */
     int32_t postpone_pct;    /* postpone a triangle group below this % of active lanes (0 = off) */
     int32_t prefetch;        /* 0 or 1: issue the next node's loads before the triangle loop */
+    int32_t work_slices;     /* 1..8 contiguous ray ranges, one drained first per XCD (CWBVH) */
 } cwbvh_trace_config;
 
 int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);

@@ -1,0 +1,336 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+Contract: with triangle postponing off (the default), every kernel variant
+returns hits bit-identical to the scalar oracle -- t, u, v and triAddr.  With
+postponing on, closest hits may differ only on grazing ties (classified by
+oracle.compare_hits) and any-hit occlusion must agree on >= 99.999% of rays.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-test] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def bits(h):
+    return np.ascontiguousarray(h, np.float32).view(np.uint32)
+
+
+def assert_bitwise(gpu, ref, what):
+    diff = (bits(gpu) != bits(ref)).any(axis=1)
+    if diff.any():
+        i = np.nonzero(diff)[0][:5]
+        raise AssertionError(f"{what}: {int(diff.sum())} of {len(ref)} rays differ; first {i.tolist()}: "
+                             f"gpu={gpu[i].tolist()} ref={ref[i].tolist()}")
+
+
+def oracle_trace(orc, lay, rays, any_hit=False, counters=False):
+    if lay.kind == 1:
+        return orc.trace_cwbvh(lay.nodes, lay.node_stride, lay.woop, rays, any_hit=any_hit, counters=counters)
+    return orc.trace_bvh2(lay.nodes, lay.woop, rays, any_hit=any_hit, counters=counters)
+
+
+@pytest.fixture(scope="module")
+def c0(cw):
+    from cwbvh_amd import workloads as W
+
+    say("building C0")
+    wl = W.c0()
+    bvh2, cwb = cw.build_layouts(wl.mesh)
+    return wl, {"bvh2": bvh2, "cwbvh": cwb}
+
+
+@pytest.fixture(scope="module")
+def c0_ref(orc, c0):
+    wl, lays = c0
+    out = {}
+    for k, lay in lays.items():
+        for any_hit in (False, True):
+            out[(k, any_hit)] = oracle_trace(orc, lay, wl.rays, any_hit)
+    return out
+
+
+@pytest.mark.parametrize("kind", ["cwbvh", "bvh2"])
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_c0_bit_identical(cw, c0, c0_ref, kind, any_hit):
+    wl, lays = c0
+    say(f"C0 {kind} any={any_hit} n={len(wl.rays)}")
+    sc = cw.Scene(lays[kind])
+    g = sc.trace_numpy(wl.rays, cw.TRACE_ANY if any_hit else cw.TRACE_CLOSEST)
+    assert_bitwise(g, c0_ref[(kind, any_hit)], f"C0 {kind} any={any_hit}")
+    assert not cw.stack_overflowed()
+
+
+CWBVH_VARIANTS = [
+    dict(block_size=64, lds_depth=8),
+    dict(block_size=256, lds_depth=4, fetch_chunk=256),
+    dict(block_size=128, lds_depth=1, fetch_chunk=1),
+    dict(block_size=128, lds_depth=32, refill_idle=0, refill_budget=1),
+    dict(block_size=64, lds_depth=64, refill_idle=64, refill_budget=1 << 20),
+    dict(block_size=128, lds_depth=16, grid_blocks=7),
+    dict(work_slices=1),
+    dict(work_slices=3, block_size=64, prefetch=1),
+    dict(prefetch=1, lds_depth=2, fetch_chunk=7),
+]
+
+
+@pytest.mark.parametrize("variant", range(len(CWBVH_VARIANTS)))
+def test_cwbvh_config_variants_bit_identical(cw, c0, c0_ref, variant):
+    wl, lays = c0
+    kw = CWBVH_VARIANTS[variant]
+    say(f"CWBVH variant {kw}")
+    cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw)
+    sc = cw.Scene(lays["cwbvh"])
+    for any_hit in (False, True):
+        g = sc.trace_numpy(wl.rays, int(any_hit), cfg)
+        assert_bitwise(g, c0_ref[("cwbvh", any_hit)], f"variant {kw} any={any_hit}")
+
+
+@pytest.mark.parametrize("kw", [dict(block_size=64, lds_depth=1), dict(block_size=256, lds_depth=8),
+                                dict(block_size=128, lds_depth=64)])
+def test_bvh2_config_variants_bit_identical(cw, c0, c0_ref, kw):
+    wl, lays = c0
+    say(f"BVH2 variant {kw}")
+    cfg = cw.default_trace_config(cw.KIND_BVH2, **kw)
+    sc = cw.Scene(lays["bvh2"])
+    for any_hit in (False, True):
+        g = sc.trace_numpy(wl.rays, int(any_hit), cfg)
+        assert_bitwise(g, c0_ref[("bvh2", any_hit)], f"bvh2 {kw} any={any_hit}")
+
+
+@pytest.mark.parametrize("kw", [dict(postpone_pct=20), dict(postpone_pct=40, prefetch=1, refill_idle=4,
+                                                           refill_budget=8)])
+def test_cwbvh_postponing_within_contract(cw, orc, c0, c0_ref, kw):
+    wl, lays = c0
+    say(f"CWBVH triangle postponing {kw}")
+    cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw)
+    lay = lays["cwbvh"]
+    sc = cw.Scene(lay)
+    g = sc.trace_numpy(wl.rays, 0, cfg)
+    r = c0_ref[("cwbvh", False)]
+    cmp = orc.compare_hits(g, r, lay.woop, wl.rays, lay.prim_idx)
+    assert not cmp["mismatches"], cmp["mismatches"][:10]
+    assert cmp["ties"] <= 1e-5 * len(r)
+    ga = sc.trace_numpy(wl.rays, 1, cfg)
+    ca = orc.compare_hits(ga, c0_ref[("cwbvh", True)], lay.woop, wl.rays, lay.prim_idx, any_hit=True)
+    assert ca["exact"] >= (1 - 1e-5) * len(r)
+    # every reported any-hit triangle is a real hit of its ray
+    ids = cw.hit_ids(ga)
+    for i in np.nonzero(ids >= 0)[0][:2000]:
+        ok, t, u, v = orc.woop_test(lay.woop[ids[i]:ids[i] + 3], wl.rays[i])
+        assert ok and np.float32(t) == ga[i, 0]
+
+
+def test_stride8_relayout_bit_identical(cw, c0, c0_ref):
+    wl, lays = c0
+    say("CWBVH 128-byte nodes")
+    sc = cw.Scene(lays["cwbvh"].restride(8))
+    for any_hit in (False, True):
+        assert_bitwise(sc.trace_numpy(wl.rays, int(any_hit)), c0_ref[("cwbvh", any_hit)], f"stride8 any={any_hit}")
+
+
+def test_stats_counters_match_oracle(cw, orc, c0):
+    wl, lays = c0
+    rays = wl.rays[:200000]
+    for k, lay in lays.items():
+        say(f"counters {k}")
+        _, rc = oracle_trace(orc, lay, rays, counters=True)
+        _, gc = cw.Scene(lay).trace_numpy(rays, 0, with_stats=True)
+        assert np.array_equal(rc, gc), (k, rc, gc)
+
+
+def test_prim_id_output(cw, c0, c0_ref):
+    wl, lays = c0
+    for k, lay in lays.items():
+        say(f"PRIM_ID {k}")
+        g = cw.Scene(lay).trace_numpy(wl.rays, cw.TRACE_PRIM_ID)
+        ids = cw.hit_ids(c0_ref[(k, False)])
+        want = np.where(ids >= 0, lay.prim_idx[np.clip(ids, 0, None)], -1)
+        assert np.array_equal(cw.hit_ids(g), want)
+        assert np.array_equal(bits(g)[:, [0, 2, 3]], bits(c0_ref[(k, False)])[:, [0, 2, 3]])
+
+
+def test_ordered_trace(cw, c0, c0_ref):
+    wl, lays = c0
+    perm = np.random.default_rng(3).permutation(len(wl.rays)).astype(np.int32)
+    for k, lay in lays.items():
+        say(f"ordered {k}")
+        g = cw.Scene(lay).trace_numpy(wl.rays, 0, order=perm)
+        assert_bitwise(g, c0_ref[(k, False)], f"ordered {k}")
+
+
+def test_known_answer_planes(cw):
+    n_tri, n_ray = 5000, 100000
+    mesh = cw.Mesh.validation_planes(n_tri)
+    rays = cw.rays_validation(n_ray, n_tri)
+    k = (np.arange(n_ray, dtype=np.uint64) * np.uint64(12345678) % np.uint64(n_tri)).astype(np.int64)
+    bvh2, cwb = cw.build_layouts(mesh)
+    for lay in (cwb, bvh2):
+        say(f"KAT kind={lay.kind}")
+        g = cw.Scene(lay).trace_numpy(rays, cw.TRACE_PRIM_ID)
+        assert np.array_equal(cw.hit_ids(g), k)
+
+
+def test_edge_cases(cw, orc):
+    rng = np.random.default_rng(11)
+    mesh = cw.Mesh.heightfield(32, 16, 8, 0.1, seed=2)
+    bvh2, cwb = cw.build_layouts(mesh)
+    rays = np.zeros((1000, 8), np.float32)
+    rays[:, :3] = rng.uniform(-0.2, 1.2, size=(1000, 3))
+    d = rng.normal(size=(1000, 3))
+    d[:100] = [0, 0, -1]            # axis-aligned
+    d[100:200] = [-0.0, 0.0, 1.0]   # signed zeros
+    d[200:300] = [1, 0, 0]          # grazing the heightfield
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 7] = np.inf             # tmax = inf
+    rays[300:400, 7] = 1e-6         # tiny tmax
+    rays[400:500, 0] += 100         # misses everything
+    for lay in (cwb, bvh2):
+        sc = cw.Scene(lay)
+        for n in (1, 63, 65, 1000):
+            say(f"edge kind={lay.kind} n={n}")
+            for any_hit in (False, True):
+                g = sc.trace_numpy(rays[:n], int(any_hit))
+                assert_bitwise(g, oracle_trace(orc, lay, rays[:n], any_hit), f"edge n={n}")
+        sc.trace(0, 0, 0)  # n == 0 is a no-op
+        cw.synchronize()
+
+
+def test_single_triangle_and_empty_scene(cw, orc):
+    one = cw.Mesh.from_arrays([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 1, 2]])
+    empty = cw.Mesh.from_arrays(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32))
+    rays = np.array([[0.2, 0.2, 1, 0, 0, 0, -1, 10], [2, 2, 1, 0, 0, 0, -1, 10]], np.float32)
+    for mesh in (one, empty):
+        for lay in cw.build_layouts(mesh):
+            say(f"tiny scene kind={lay.kind}")
+            g = cw.Scene(lay).trace_numpy(rays)
+            assert_bitwise(g, oracle_trace(orc, lay, rays), "tiny scene")
+
+
+@pytest.fixture(scope="module")
+def c1_small(cw, orc):
+    from cwbvh_amd import workloads as W
+
+    say("building C1 (980k triangles)")
+    mesh = W.c1_mesh()
+    bvh2, cwb = cw.build_layouts(mesh)
+
+    def primary(m, rays):
+        h = orc.trace_cwbvh(cwb.nodes, 5, cwb.woop, rays)
+        ids = orc.hit_ids(h)
+        return h, np.where(ids >= 0, cwb.prim_idx[np.clip(ids, 0, None)], -1).astype(np.int32)
+
+    wl = W.c1(primary, width=320, height=180, mesh=mesh)
+    return wl, {"bvh2": bvh2, "cwbvh": cwb}
+
+
+@pytest.mark.parametrize("kind", ["cwbvh", "bvh2"])
+def test_c1_sample_bit_identical(cw, orc, c1_small, kind):
+    wl, lays = c1_small
+    lay = lays[kind]
+    sc = cw.Scene(lay)
+    for name, rays in (("primary", wl.extra["primary"]), ("ao", wl.rays)):
+        for any_hit in (False, True):
+            say(f"C1 {kind} {name} any={any_hit} n={len(rays)}")
+            g = sc.trace_numpy(rays, int(any_hit))
+            assert_bitwise(g, oracle_trace(orc, lay, rays, any_hit), f"C1 {kind} {name}")
+
+
+def test_compat_entry_points(cw, orc, c0, c0_ref):
+    wl, lays = c0
+    rays = wl.rays
+    n = len(rays)
+    d_rays = cw.DeviceBuffer.from_numpy(rays)
+    d_hits = cw.DeviceBuffer(16 * n)
+    bufs = {}
+    for k, lay in lays.items():
+        bufs[k] = [cw.DeviceBuffer.from_numpy(lay.nodes), cw.DeviceBuffer.from_numpy(lay.woop),
+                   cw.DeviceBuffer.from_numpy(lay.prim_idx)]
+    say("rtBindCWBVHData / rtTraceCWBVH")
+    cw.rtBindCWBVHData(*(b.ptr for b in bufs["cwbvh"]))
+    ms = cw.rtTraceCWBVH(d_rays.ptr, d_hits.ptr, n)
+    assert ms > 0
+    assert_bitwise(d_hits.download((n, 4)), c0_ref[("cwbvh", False)], "rtTraceCWBVH")
+    say("rtBindBVH2Data / rtTraceBVH2")
+    cw.rtBindBVH2Data(*(b.ptr for b in bufs["bvh2"]))
+    cw.rtTraceBVH2(d_rays.ptr, d_hits.ptr, n)
+    assert_bitwise(d_hits.download((n, 4)), c0_ref[("bvh2", False)], "rtTraceBVH2")
+    # rebinding replaces the previous binding
+    cw.rtBindCWBVHData(*(b.ptr for b in bufs["cwbvh"]))
+    cw.rtTraceCWBVH(d_rays.ptr, d_hits.ptr, 1000)
+    assert_bitwise(d_hits.download((1000, 4)), c0_ref[("cwbvh", False)][:1000], "rebind")
+
+
+def deep_chain_cwbvh(cw, depth):
+    """CWBVH whose every node has two inner children covering the same box, the
+    chain child visited first: the sibling is pushed at every level, so a ray
+    needs `depth` stack entries."""
+    def node(p, e, imask, base, tbase, metas, q_lo, q_hi):
+        n = np.zeros((5, 4), np.float32)
+        n[0, :3] = p
+        b = n[0].view(np.uint8)
+        b[12:15] = np.array(e, np.int8).view(np.uint8)
+        b[15] = imask
+        n[1].view(np.int32)[0] = base
+        n[1].view(np.int32)[1] = tbase
+        n[1].view(np.uint8)[8:16] = metas
+        q = n[2:5].view(np.uint8).reshape(-1)
+        for s in range(8):
+            if metas[s]:
+                q[0 * 8 + s] = q[1 * 8 + s] = q[2 * 8 + s] = q_lo
+                q[24 + 0 * 8 + s] = q[24 + 8 + s] = q[24 + 16 + s] = q_hi
+        return n
+
+    nodes = {}
+    e = [-7, -7, -7]  # 255 * 2^-7 covers [0, 1.99]
+    inner0 = (1 << 5) | 24
+    inner1 = (1 << 5) | 25
+    leaf1 = (1 << 5) | 0
+    for i in range(depth):
+        idx = 0 if i == 0 else 2 * i - 1
+        if i < depth - 1:
+            nodes[idx] = node((0, 0, 0), e, 0b11, 2 * i + 1, 0, [inner0, inner1, 0, 0, 0, 0, 0, 0], 0, 255)
+            nodes[2 * i + 2] = node((0, 0, 0), e, 0, 0, i + 1, [leaf1, 0, 0, 0, 0, 0, 0, 0], 0, 255)
+        else:
+            nodes[idx] = node((0, 0, 0), e, 0, 0, 0, [leaf1, 0, 0, 0, 0, 0, 0, 0], 0, 255)
+    arr = np.concatenate([nodes[i] for i in range(max(nodes) + 1)])
+    far = cw.woopify([1000, 0, 0], [1001, 0, 0], [1000, 1, 0])
+    woop = np.concatenate([far] * (depth + 1)).astype(np.float32)
+    prim = np.tile(np.array([0, -1, -1], np.int32), depth + 1)
+    return arr, woop, prim
+
+
+def test_stack_overflow_is_flagged_not_faulted(cw, orc):
+    ray = np.array([[-1, -1, -1, 0, *(np.ones(3) / np.sqrt(3)), 1e30]], np.float32)
+    # within capacity: traversal equals the oracle
+    nodes, woop, prim = deep_chain_cwbvh(cw, 40)
+    lay = cw.Layout.from_arrays(cw.KIND_CWBVH, 5, nodes, woop, prim)
+    assert lay.stats()["stack_need"] == 41
+    say("deep chain within capacity")
+    g, cnt = cw.Scene(lay).trace_numpy(np.repeat(ray, 64, axis=0), 0, with_stats=True)
+    r, rc = orc.trace_cwbvh(nodes, 5, woop, np.repeat(ray, 64, axis=0), counters=True)
+    assert_bitwise(g, r, "deep chain")
+    assert np.array_equal(cnt, rc) and rc[0] == 64 * (2 * 40 - 1)
+    assert not cw.stack_overflowed()
+    # deeper than the kernels' 64 entries: scene creation refuses it ...
+    nodes, woop, prim = deep_chain_cwbvh(cw, 100)
+    lay = cw.Layout.from_arrays(cw.KIND_CWBVH, 5, nodes, woop, prim)
+    with pytest.raises(cw.CwbvhError) as e:
+        cw.Scene(lay)
+    assert e.value.status == 4
+    # ... and the unchecked bind path reports the overflow instead of faulting
+    say("deep chain beyond capacity through rtBindCWBVHData")
+    dn, dw, dp = (cw.DeviceBuffer.from_numpy(a) for a in (nodes, woop, prim))
+    dr = cw.DeviceBuffer.from_numpy(np.repeat(ray, 64, axis=0))
+    dh = cw.DeviceBuffer(16 * 64)
+    cw.rtBindCWBVHData(dn.ptr, dw.ptr, dp.ptr)
+    with pytest.raises(cw.CwbvhError) as e:
+        cw.rtTraceCWBVH(dr.ptr, dh.ptr, 64)
+    assert e.value.status == 4
+    assert not cw.stack_overflowed()  # the flag was consumed by the failing call
