@@ -78,12 +78,200 @@ def cpu_baseline(layout, rays, target_s: float, threads: int) -> dict:
                       f"(oracle/oracle.c) on {threads} host threads, {dt:.1f} s wall"}
 
 
+def bench_c2(args, cw, W, dev) -> int:
+    """C2: many scenes, uneven AO batches.  Plain = one cwbvh_trace launch per
+    scene (default configuration) back to back on one stream; batched = one
+    autotuned persistent multi-scene launch replayed from a hipGraph (the
+    Morton pre-pass, when the tuner picks it, is inside the timed region)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    plan = W.c2_plan(num_scenes=args.c2_scenes)
+    t0 = time.perf_counter()
+
+    def make(entry):
+        mesh, rays, _ = W.c2_scene(entry)
+        _, lay = cw.build_layouts(mesh, cw.default_build_params(num_threads=2), bvh2=False)
+        return entry, mesh.num_triangles, lay, rays
+
+    with ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1) // 2)) as ex:
+        built = list(ex.map(make, plan))
+    build_s = time.perf_counter() - t0
+    tris = sum(b[1] for b in built)
+    log(f"C2: {len(built)} scenes, {tris} triangles, {sum(len(b[3]) for b in built)} rays, build {build_s:.1f} s")
+    scenes, bufs, items = [], [], []
+    for entry, nt, lay, rays in built:
+        sc = cw.Scene(lay)
+        dr = cw.DeviceBuffer.from_numpy(rays)
+        dh = cw.DeviceBuffer(16 * len(rays))
+        scenes.append(sc)
+        bufs.append((dr, dh))
+        items.append((sc, dr.ptr, dh.ptr, len(rays)))
+    del built
+    total = sum(it[3] for it in items)
+    st = cw.Stream()
+    cfg0 = cw.default_trace_config(cw.KIND_CWBVH)
+
+    def plain():
+        for sc, r, h, n in items:
+            sc.trace(r, h, n, 0, cfg0, None, st)
+
+    def plain_sync():
+        for sc, r, h, n in items:
+            sc.trace(r, h, n, 0, cfg0, None, st)
+            st.synchronize()
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        st.synchronize()
+        e0, e1 = cw.Event(), cw.Event()
+        e0.record(st)
+        for _ in range(steps):
+            fn()
+        e1.record(st)
+        e1.synchronize()
+        return e0.elapsed_ms(e1) / steps
+
+    ms_plain = timed(plain, args.steps, args.warmup)
+    t = time.perf_counter()
+    for _ in range(max(1, args.steps // 2)):
+        plain_sync()
+    ms_plain_sync = (time.perf_counter() - t) * 1e3 / max(1, args.steps // 2)
+    log(f"plain {ms_plain:.2f} ms, plain+sync {ms_plain_sync:.2f} ms")
+    t = time.perf_counter()
+    best, recs = cw.autotune_batch(items, block_sizes=(64, 128, 256), lds_depths=(8, 16, 32), morton=(0, 1),
+                                   prefetch=(0, 1), reps=args.tune_reps, stream=st)
+    tune_s = time.perf_counter() - t
+    log(f"autotune {tune_s:.1f} s best {best.as_dict()}")
+    batch = cw.Batch(items, cw.TRACE_CLOSEST, best.cfg, morton=bool(best.morton))
+    batch.capture(st)
+    ms_batch = timed(lambda: batch.replay(st), args.steps, args.warmup)
+    untuned = cw.Batch(items, cw.TRACE_CLOSEST, cfg0, morton=False)
+    ms_untuned = timed(lambda: untuned.launch(st), args.steps, args.warmup)
+    value = total / (ms_batch / 1e3) / 1e6
+    out = {
+        "metric": METRIC + " [C2 batched]",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_batch, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"C2: {len(items)} seeded scenes (heightfields / icosphere lattices), "
+                               f"{tris} triangles, {total} AO rays (closest hit)",
+                   "triangles": tris, "rays": total, "scenes": len(items), "seed": W.SEEDS["C2"]},
+        "plain_ms": round(ms_plain, 4),
+        "plain_mrays_s": round(total / ms_plain / 1e3, 2),
+        "plain_sync_ms": round(ms_plain_sync, 4),
+        "batched_untuned_ms": round(ms_untuned, 4),
+        "speedup_vs_plain": round(ms_plain / ms_batch, 3),
+        "speedup_vs_plain_sync": round(ms_plain_sync / ms_batch, 3),
+        "autotune": {"best": best.as_dict(), "seconds": round(tune_s, 1),
+                     "records": [r.as_dict() for r in recs]},
+        "build_s": round(build_s, 1),
+        "device": dev["arch"],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def bench_c4(args, cw, W, dev) -> int:
+    """C4: resident scenes, Poisson AO query requests (log-uniform 1k..1M rays,
+    uniform scene), host-resident rays copied in and hits copied out per
+    request.  Continuous batching vs serial per-request plain launches."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    specs = W.c4_plan(num_scenes=args.c4_scenes, total_tris=args.c4_tris)
+    t0 = time.perf_counter()
+
+    def make(entry):
+        mesh = W.c4_mesh(entry)
+        _, lay = cw.build_layouts(mesh, cw.default_build_params(num_threads=4), bvh2=False)
+        diag = mesh.diagonal()
+        rays = cw.rays_ao_surface(mesh, 131072, 8, 1e-4 * diag, 0.0, 0.1 * diag, seed=entry["seed"] + 1)
+        return entry, mesh.num_triangles, lay, rays
+
+    with ThreadPoolExecutor(max_workers=max(1, min(16, os.cpu_count() or 1) // 4)) as ex:
+        built = list(ex.map(make, specs))
+    build_s = time.perf_counter() - t0
+    tris = sum(b[1] for b in built)
+    scenes, pools, hbm = [], [], 0
+    for entry, nt, lay, rays in built:
+        sc = cw.Scene(lay)
+        hbm += sc.device_bytes()
+        pool = cw.PinnedArray(rays.shape, np.float32)
+        pool.array[:] = rays
+        scenes.append(sc)
+        pools.append(pool)
+    del built
+    log(f"C4: {len(scenes)} resident scenes, {tris} triangles, {hbm / 2**30:.2f} GiB of BVH in HBM, "
+        f"build {build_s:.1f} s")
+    hit_pool = cw.PinnedArray((16 << 20, 4), np.float32)
+    rates = [float(r) for r in args.c4_rates.split(",")]
+    table = []
+    for serial in (0, 1):
+        cfg = cw.default_server_config(serial=serial, memory_budget=8 << 30, max_inflight=3,
+                                       max_batch_requests=64, max_batch_rays=16 << 20)
+        srv = cw.Server(scenes, cfg)
+        srv.run_poisson(2000.0, 20, 1000, 1_000_000, pools, hit_pool, seed=1)  # warm-up
+        for rate in rates:
+            res, _ = srv.run_poisson(rate, args.c4_requests, 1000, 1_000_000, pools, hit_pool, seed=int(rate))
+            st = srv.stats()
+            row = {"mode": "serial" if serial else "continuous", "rate_req_s": rate,
+                   "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
+                   "mean_ms": round(res["mean_ms"], 3), "mrays_per_s": round(res["mrays_per_s"], 1),
+                   "wall_s": round(res["wall_s"], 3), "max_inflight_seen": st["max_inflight_seen"],
+                   "max_batch_requests_seen": st["max_batch_requests_seen"]}
+            log(str(row))
+            table.append(row)
+        srv.close()
+    cont = [r for r in table if r["mode"] == "continuous"]
+    ser = [r for r in table if r["mode"] == "serial"]
+    out = {
+        "metric": METRIC + " [C4 serving]",
+        "value": cont[0]["p99_ms"],
+        "unit": "ms (p99 latency at the lowest rate)",
+        "n_gpus": 1,
+        "steps": args.c4_requests,
+        "warmup": 20,
+        "ms_per_step": cont[0]["mean_ms"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"C4: {len(scenes)} resident scenes ({tris} triangles), Poisson requests of "
+                               f"log-uniform 1k..1M AO rays, host rays in / hits out (PCIe included)",
+                   "triangles": tris, "scenes": len(scenes), "requests_per_rate": args.c4_requests,
+                   "rates": rates, "seed": W.SEEDS["C4"]},
+        "table": table,
+        "max_mrays_per_s": {"continuous": max(r["mrays_per_s"] for r in cont),
+                            "serial": max(r["mrays_per_s"] for r in ser)},
+        "bvh_hbm_gib": round(hbm / 2**30, 2),
+        "build_s": round(build_s, 1),
+        "device": dev["arch"],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--c4-scenes", type=int, default=32)
+    ap.add_argument("--c4-tris", type=int, default=100_000_000, help="total triangles of the C4 scene set")
+    ap.add_argument("--c4-requests", type=int, default=400)
+    ap.add_argument("--c4-rates", default="1000,2000,5000,10000,20000,50000")
+    ap.add_argument("--c2-scenes", type=int, default=256)
+    ap.add_argument("--tune-reps", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--node-stride", type=int, default=5, choices=[5, 8])
@@ -103,6 +291,11 @@ def main() -> int:
         return 3
     dev = cw.device_info(0)
     log(f"device {dev['arch']} CUs={dev['compute_units']}")
+
+    if args.config == "c2":
+        return bench_c2(args, cw, W, dev)
+    if args.config == "c4":
+        return bench_c4(args, cw, W, dev)
 
     params = cw.default_build_params(cwbvh_node_stride=args.node_stride)
     t0 = time.perf_counter()

@@ -84,7 +84,57 @@ class TraceConfig(C.Structure):
         ("postpone_pct", C.c_int32),
         ("prefetch", C.c_int32),
         ("work_slices", C.c_int32),
+        ("persistent", C.c_int32),
     ]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class BatchItem(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("d_rays", C.c_void_p), ("d_hits", C.c_void_p), ("n", C.c_int64)]
+
+
+class TuneSpace(C.Structure):
+    _fields_ = [
+        ("block_sizes", C.c_int32 * 4), ("n_block", C.c_int32),
+        ("lds_depths", C.c_int32 * 4), ("n_lds", C.c_int32),
+        ("morton", C.c_int32 * 2), ("n_morton", C.c_int32),
+        ("prefetch", C.c_int32 * 2), ("n_prefetch", C.c_int32),
+    ]
+
+
+class TuneRecord(C.Structure):
+    _fields_ = [("cfg", TraceConfig), ("morton", C.c_int32), ("ms", C.c_float)]
+
+    def as_dict(self) -> dict:
+        d = self.cfg.as_dict()
+        d.update(morton=self.morton, ms=self.ms)
+        return d
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("memory_budget", C.c_int64), ("max_inflight", C.c_int32), ("max_batch_requests", C.c_int32),
+                ("max_batch_rays", C.c_int64), ("serial", C.c_int32), ("flags", C.c_uint32), ("trace", TraceConfig)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_int64), ("rays", C.c_int64), ("batches", C.c_int64),
+                ("max_inflight_seen", C.c_int32), ("max_batch_requests_seen", C.c_int32),
+                ("peak_bytes", C.c_int64), ("busy_s", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class PoissonSpec(C.Structure):
+    _fields_ = [("rate", C.c_double), ("num_requests", C.c_int32), ("min_rays", C.c_int64),
+                ("max_rays", C.c_int64), ("seed", C.c_uint64)]
+
+
+class PoissonResult(C.Structure):
+    _fields_ = [("wall_s", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("max_ms", C.c_double), ("rays", C.c_int64), ("mrays_per_s", C.c_double)]
 
     def as_dict(self) -> dict:
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -166,6 +216,25 @@ _SIGS = {
     "cwbvh_trace": (_I32, [_P, _P, _P, _I64, _U32, _P, _P, _P]),
     "cwbvh_trace_ordered": (_I32, [_P, _P, _P, _P, _I64, _U32, _P, _P, _P]),
     "cwbvh_stack_overflowed": (_I32, [_P, C.POINTER(_I32)]),
+    "cwbvh_scene_bounds": (_I32, [_P, _P, _P]),
+    "cwbvh_morton_order": (_I32, [_P, _I64, _P, _P, _P, _P]),
+    "cwbvh_batch_create": (_I32, [_P, _I32, _U32, _P, _I32, _PP]),
+    "cwbvh_batch_launch": (_I32, [_P, _P]),
+    "cwbvh_batch_capture": (_I32, [_P, _P]),
+    "cwbvh_batch_replay": (_I32, [_P, _P]),
+    "cwbvh_batch_info": (_I32, [_P, C.POINTER(_I64), C.POINTER(_I32), C.POINTER(_I64)]),
+    "cwbvh_batch_destroy": (None, [_P]),
+    "cwbvh_autotune_batch": (_I32, [_P, _I32, _U32, _P, _I32, _P, _I32, C.POINTER(_I32), _P, _P]),
+    "cwbvh_host_alloc": (_I32, [_PP, _I64]),
+    "cwbvh_host_free": (_I32, [_P]),
+    "cwbvh_server_config_default": (_I32, [_P]),
+    "cwbvh_server_create": (_I32, [_P, _I32, _P, _PP]),
+    "cwbvh_server_submit": (_I32, [_P, _I32, _P, _I64, _P, C.POINTER(_U64)]),
+    "cwbvh_server_wait": (_I32, [_P, _U64, C.c_double, C.POINTER(C.c_double)]),
+    "cwbvh_server_drain": (_I32, [_P]),
+    "cwbvh_server_get_stats": (_I32, [_P, _P]),
+    "cwbvh_server_destroy": (None, [_P]),
+    "cwbvh_server_run_poisson": (_I32, [_P, _P, _P, _P, _P, _I64, _P, _P]),
     "rtBindCWBVHData": (None, [_P, _P, _P]),
     "rtTraceCWBVH": (None, [_P, _P, C.c_int]),
     "rtBindBVH2Data": (None, [_P, _P, _P]),
@@ -581,6 +650,12 @@ class Scene:
         check(lib.cwbvh_scene_device_bytes(self._s, C.byref(b)))
         return b.value
 
+    def bounds(self) -> tuple[np.ndarray, np.ndarray]:
+        lo = np.zeros(3, np.float32)
+        hi = np.zeros(3, np.float32)
+        check(lib.cwbvh_scene_bounds(self._s, _ptr(lo), _ptr(hi)))
+        return lo, hi
+
     def trace(self, d_rays: int, d_hits: int, n: int, flags: int = TRACE_CLOSEST, cfg: TraceConfig | None = None,
               d_counters: int | None = None, stream: Stream | None = None, d_order: int | None = None) -> None:
         cfgp = C.byref(cfg) if cfg is not None else None
@@ -615,6 +690,154 @@ class Scene:
         if s is not None and s.value:
             lib.cwbvh_scene_destroy(s)
             self._s = C.c_void_p(0)
+
+
+def morton_order(d_rays: int, n: int, lo, hi, d_order: int, stream: Stream | None = None) -> None:
+    lo = _f32(lo)
+    hi = _f32(hi)
+    check(lib.cwbvh_morton_order(d_rays, n, _ptr(lo), _ptr(hi), d_order, stream.handle if stream else None))
+
+
+def _items(items) -> C.Array:
+    arr = (BatchItem * max(1, len(items)))()
+    for i, (scene, d_rays, d_hits, n) in enumerate(items):
+        arr[i] = BatchItem(scene.handle.value, d_rays, d_hits, n)
+    return arr
+
+
+class Batch:
+    """One persistent multi-scene launch over items [(scene, d_rays, d_hits, n), ...]."""
+
+    def __init__(self, items, flags: int = TRACE_CLOSEST, cfg: TraceConfig | None = None, morton: bool = False):
+        self._keep = [it[0] for it in items]
+        arr = _items(items)
+        b = C.c_void_p()
+        check(lib.cwbvh_batch_create(arr, len(items), flags, C.byref(cfg) if cfg is not None else None,
+                                     1 if morton else 0, C.byref(b)))
+        self._b = b
+
+    def launch(self, stream: Stream | None = None) -> None:
+        check(lib.cwbvh_batch_launch(self._b, stream.handle if stream else None))
+
+    def capture(self, stream: Stream) -> None:
+        check(lib.cwbvh_batch_capture(self._b, stream.handle))
+
+    def replay(self, stream: Stream | None = None) -> None:
+        check(lib.cwbvh_batch_replay(self._b, stream.handle if stream else None))
+
+    def info(self) -> dict:
+        r, j, b = C.c_int64(), C.c_int32(), C.c_int64()
+        check(lib.cwbvh_batch_info(self._b, C.byref(r), C.byref(j), C.byref(b)))
+        return {"total_rays": r.value, "num_jobs": j.value, "device_bytes": b.value}
+
+    def __del__(self):
+        b = getattr(self, "_b", None)
+        if b is not None and b.value:
+            lib.cwbvh_batch_destroy(b)
+            self._b = C.c_void_p(0)
+
+
+def autotune_batch(items, flags: int = TRACE_CLOSEST, block_sizes=(64, 128, 256), lds_depths=(8, 16, 32),
+                   morton=(0, 1), prefetch=(0,), reps: int = 3, stream: Stream | None = None):
+    """Time every configuration of the space on the batch; returns (best, records)."""
+    sp = TuneSpace()
+    for name, vals, cap in (("block_sizes", block_sizes, 4), ("lds_depths", lds_depths, 4),
+                            ("morton", morton, 2), ("prefetch", prefetch, 2)):
+        assert 1 <= len(vals) <= cap
+        arr = getattr(sp, name)
+        for i, v in enumerate(vals):
+            arr[i] = v
+    sp.n_block, sp.n_lds, sp.n_morton, sp.n_prefetch = len(block_sizes), len(lds_depths), len(morton), len(prefetch)
+    cap = 4 * 4 * 2 * 2
+    recs = (TuneRecord * cap)()
+    nrec = C.c_int32()
+    best = TuneRecord()
+    arr = _items(items)
+    check(lib.cwbvh_autotune_batch(arr, len(items), flags, C.byref(sp), reps, recs, cap, C.byref(nrec),
+                                   C.byref(best), stream.handle if stream else None))
+    return best, [recs[i] for i in range(nrec.value)]
+
+
+class PinnedArray:
+    """Pinned host allocation exposed as a numpy array (freed with the object)."""
+
+    def __init__(self, shape, dtype=np.float32):
+        dtype = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dtype.itemsize
+        p = C.c_void_p()
+        check(lib.cwbvh_host_alloc(C.byref(p), max(nbytes, 1)))
+        self._p = p
+        buf = (C.c_char * max(nbytes, 1)).from_address(p.value)
+        self.array = np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+    @property
+    def ptr(self) -> int:
+        return self._p.value
+
+    def __del__(self):
+        p = getattr(self, "_p", None)
+        if p is not None and p.value:
+            self.array = None
+            lib.cwbvh_host_free(p)
+            self._p = C.c_void_p(0)
+
+
+def default_server_config(**overrides) -> ServerConfig:
+    c = ServerConfig()
+    check(lib.cwbvh_server_config_default(C.byref(c)))
+    for k, v in overrides.items():
+        setattr(c, k, v)
+    return c
+
+
+class Server:
+    """Continuous-batching AO query server over resident CWBVH scenes."""
+
+    def __init__(self, scenes, cfg: ServerConfig | None = None):
+        self._scenes = list(scenes)
+        arr = (C.c_void_p * len(self._scenes))(*[sc.handle.value for sc in self._scenes])
+        s = C.c_void_p()
+        check(lib.cwbvh_server_create(arr, len(self._scenes), C.byref(cfg) if cfg is not None else None, C.byref(s)))
+        self._s = s
+
+    def submit(self, scene: int, host_rays_ptr: int, n: int, host_hits_ptr: int) -> int:
+        rid = C.c_uint64()
+        check(lib.cwbvh_server_submit(self._s, scene, host_rays_ptr, n, host_hits_ptr, C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int, timeout_s: float = -1.0) -> float:
+        lat = C.c_double()
+        check(lib.cwbvh_server_wait(self._s, rid, timeout_s, C.byref(lat)))
+        return lat.value
+
+    def drain(self) -> None:
+        check(lib.cwbvh_server_drain(self._s))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib.cwbvh_server_get_stats(self._s, C.byref(st)))
+        return st.as_dict()
+
+    def run_poisson(self, rate: float, num_requests: int, min_rays: int, max_rays: int, ray_pools, hit_pool,
+                    seed: int = 0):
+        """ray_pools: list of PinnedArray (n,8) per scene; hit_pool: PinnedArray (m,4)."""
+        spec = PoissonSpec(rate, num_requests, min_rays, max_rays, seed)
+        pools = (C.c_void_p * len(ray_pools))(*[p.ptr for p in ray_pools])
+        sizes = (C.c_int64 * len(ray_pools))(*[len(p.array) for p in ray_pools])
+        lat = np.zeros(num_requests, np.float64)
+        res = PoissonResult()
+        check(lib.cwbvh_server_run_poisson(self._s, C.byref(spec), pools, sizes, hit_pool.ptr, len(hit_pool.array),
+                                           lat.ctypes.data, C.byref(res)))
+        return res.as_dict(), lat
+
+    def close(self) -> None:
+        s = getattr(self, "_s", None)
+        if s is not None and s.value:
+            lib.cwbvh_server_destroy(s)
+            self._s = C.c_void_p(0)
+
+    def __del__(self):
+        self.close()
 
 
 def stack_overflowed(stream: Stream | None = None) -> bool:

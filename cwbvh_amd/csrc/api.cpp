@@ -1,6 +1,8 @@
 // C ABI: errors, device plumbing, device scenes and the trace entry points.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <map>
 #include <mutex>
@@ -208,6 +210,45 @@ int32_t cwbvh_event_elapsed_ms(void* start, void* stop, float* ms) {
 
 // ------------------------------------------------------------ scenes
 
+// Scene bounds from the root node: decoded child boxes (CWBVH) or the two
+// child boxes (BVH2); empty or inverted boxes are skipped.
+static void layout_bounds(const cwbvh_layout& L, float lo[3], float hi[3]) {
+    float a[3] = {INFINITY, INFINITY, INFINITY}, b[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (L.kind == CWBVH_KIND_CWBVH && L.nodes.size() >= 5) {
+        const uint8_t* n0 = reinterpret_cast<const uint8_t*>(&L.nodes[0]);
+        const uint8_t* meta = reinterpret_cast<const uint8_t*>(&L.nodes[1]) + 8;
+        const uint8_t* q = reinterpret_cast<const uint8_t*>(&L.nodes[2]);
+        const float p[3] = {L.nodes[0].x, L.nodes[0].y, L.nodes[0].z};
+        for (int s = 0; s < 8; ++s) {
+            if (!meta[s]) continue;
+            for (int k = 0; k < 3; ++k) {
+                const double sc = std::ldexp(1.0, (int)(int8_t)n0[12 + k]);
+                a[k] = std::min(a[k], (float)(p[k] + q[k * 8 + s] * sc));
+                b[k] = std::max(b[k], (float)(p[k] + q[24 + k * 8 + s] * sc));
+            }
+        }
+    } else if (L.kind == CWBVH_KIND_BVH2 && L.nodes.size() >= 4) {
+        const cwbvh_float4* n = L.nodes.data();
+        const float blo[2][3] = {{n[0].x, n[0].z, n[2].x}, {n[1].x, n[1].z, n[2].z}};
+        const float bhi[2][3] = {{n[0].y, n[0].w, n[2].y}, {n[1].y, n[1].w, n[2].w}};
+        for (int c = 0; c < 2; ++c) {
+            if (!(blo[c][0] <= bhi[c][0] && blo[c][1] <= bhi[c][1] && blo[c][2] <= bhi[c][2])) continue;
+            for (int k = 0; k < 3; ++k) {
+                a[k] = std::min(a[k], blo[c][k]);
+                b[k] = std::max(b[k], bhi[c][k]);
+            }
+        }
+    }
+    for (int k = 0; k < 3; ++k) {
+        if (!(a[k] <= b[k])) {
+            a[k] = 0.f;
+            b[k] = 1.f;
+        }
+        lo[k] = a[k];
+        hi[k] = b[k];
+    }
+}
+
 int32_t cwbvh_scene_create(const cwbvh_layout* L, void* stream, cwbvh_scene** out) {
     CW_CHECK_ARG(L != nullptr && out != nullptr, "NULL argument");
     CW_CHECK_ARG(!L->nodes.empty(), "empty node array");
@@ -222,6 +263,7 @@ int32_t cwbvh_scene_create(const cwbvh_layout* L, void* stream, cwbvh_scene** ou
     s->node_stride = L->node_stride;
     s->stack_need = L->stack_need;
     s->owning = true;
+    layout_bounds(*L, s->lo, s->hi);
     s->woop_f4 = (int64_t)L->woop.size();
     hipGetDevice(&s->device);
     const size_t nb = L->nodes.size() * sizeof(cwbvh_float4);
@@ -320,12 +362,13 @@ int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* c) {
         c->block_size = 128;
         c->lds_depth = 16;
         c->grid_blocks = 0;
-        c->fetch_chunk = 64;
+        c->fetch_chunk = 0;   // automatic
         c->refill_idle = 8;
         c->refill_budget = 32;
         c->postpone_pct = 0;
         c->prefetch = 0;
         c->work_slices = 8;
+        c->persistent = 1;
     } else {
         c->block_size = 128;
         c->lds_depth = 16;

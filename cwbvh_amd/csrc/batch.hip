@@ -1,0 +1,419 @@
+// Multi-scene batches, Morton ray ordering, hipGraph capture and the
+// autotuner.
+//
+// No counterpart in the reference (which binds one scene per kernel in
+// module globals, src/TraversalKernelCWBVH.cu:14-16, and traces it with one
+// synchronous launch).  Here one persistent launch of the CWBVH kernel walks a
+// device job table: the global chunk counter hands out `fetch_chunk`-ray
+// chunks of any job to any wave (see cwbvh_trace<.., MULTI=true> in
+// trace.hip), so batches of very uneven size share one launch without tails
+// per scene.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <vector>
+
+#include "device_common.h"
+#include "internal.h"
+
+using namespace cwd;
+
+namespace {
+
+// 10-bit per axis Morton code (bit interleave by magic masks).
+__device__ __forceinline__ unsigned spread10(unsigned v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+__device__ __forceinline__ unsigned morton30(float3 p, float3 lo, float3 scale) {
+    const float fx = fminf(fmaxf((p.x - lo.x) * scale.x, 0.f), 1023.f);
+    const float fy = fminf(fmaxf((p.y - lo.y) * scale.y, 0.f), 1023.f);
+    const float fz = fminf(fmaxf((p.z - lo.z) * scale.z, 0.f), 1023.f);
+    return (spread10((unsigned)fx) << 2) | (spread10((unsigned)fy) << 1) | spread10((unsigned)fz);
+}
+
+__global__ void morton_keys_single(const Ray4* __restrict__ rays, int n, float3 lo, float3 scale,
+                                   unsigned* __restrict__ keys, int* __restrict__ vals) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 o = rays[i].o;
+    keys[i] = morton30(make_float3(o.x, o.y, o.z), lo, scale);
+    vals[i] = i;
+}
+
+struct MortonJob {
+    const Ray4* rays;
+    long long first;    // global index of the job's first ray
+    int n;
+    float lo[3];
+    float scale[3];
+};
+
+// One thread per ray of the whole batch; key = job index (high bits) : Morton.
+__global__ void morton_keys_batch(const MortonJob* __restrict__ jobs, int num_jobs, long long total,
+                                  unsigned long long* __restrict__ keys, int* __restrict__ vals) {
+    const long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    int lo = 0, hi = num_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].first <= g) lo = mid;
+        else hi = mid - 1;
+    }
+    const MortonJob& J = jobs[lo];
+    const int i = (int)(g - J.first);
+    const float4 o = J.rays[i].o;
+    const unsigned m = morton30(make_float3(o.x, o.y, o.z), make_float3(J.lo[0], J.lo[1], J.lo[2]),
+                                make_float3(J.scale[0], J.scale[1], J.scale[2]));
+    keys[g] = ((unsigned long long)lo << 30) | m;
+    vals[g] = i;
+}
+
+float3 scale_of(const float* lo, const float* hi) {
+    float s[3];
+    for (int k = 0; k < 3; ++k) {
+        const float ext = hi[k] - lo[k];
+        s[k] = ext > 0.f ? 1024.f / ext : 0.f;
+    }
+    return make_float3(s[0], s[1], s[2]);
+}
+
+int bits_for(int count) {
+    int b = 0;
+    while ((1 << b) < count) ++b;
+    return b;
+}
+
+}  // namespace
+
+struct cwbvh_batch {
+    std::vector<cw::BatchJob> jobs;
+    std::vector<int64_t> ray_first;   // per job, in the concatenated ray space
+    cw::BatchJob* d_jobs = nullptr;
+    unsigned* counters = nullptr;
+    uint32_t total_chunks = 0;
+    int64_t total_rays = 0;
+    uint32_t flags = 0;
+    cwbvh_trace_config cfg{};
+    int32_t morton = 0;
+    // Morton pre-pass buffers
+    MortonJob* d_mjobs = nullptr;
+    unsigned long long* d_keys[2] = {nullptr, nullptr};
+    int32_t* d_vals[2] = {nullptr, nullptr};
+    void* d_sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int key_bits = 30;
+    int64_t bytes = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int32_t batch_sort(cwbvh_batch* b, hipStream_t st) {
+    if (!b->morton || b->total_rays == 0) return CWBVH_OK;
+    const int block = 256;
+    const long long grid = (b->total_rays + block - 1) / block;
+    hipLaunchKernelGGL(morton_keys_batch, dim3((unsigned)grid), dim3(block), 0, st, b->d_mjobs, (int)b->jobs.size(),
+                       (long long)b->total_rays, b->d_keys[0], b->d_vals[0]);
+    CW_HIP(hipGetLastError());
+    size_t tmp = b->sort_tmp_bytes;
+    CW_HIP(hipcub::DeviceRadixSort::SortPairs(b->d_sort_tmp, tmp, b->d_keys[0], b->d_keys[1], b->d_vals[0],
+                                              b->d_vals[1], (int)b->total_rays, 0, b->key_bits, st));
+    return CWBVH_OK;
+}
+
+int32_t batch_trace(cwbvh_batch* b, hipStream_t st) {
+    if (b->total_chunks == 0) return CWBVH_OK;
+    cw::BatchLaunch L{};
+    L.d_jobs = b->d_jobs;
+    L.num_jobs = (int32_t)b->jobs.size();
+    L.total_chunks = b->total_chunks;
+    L.flags = b->flags;
+    L.cfg = b->cfg;
+    L.stats = nullptr;
+    L.counters = b->counters;
+    L.stream = st;
+    return cw::launch_cwbvh_batch(L);
+}
+
+}  // namespace
+
+using namespace cw;
+
+extern "C" {
+
+int32_t cwbvh_scene_bounds(const cwbvh_scene* s, float* lo3, float* hi3) {
+    CW_CHECK_ARG(s && lo3 && hi3, "NULL argument");
+    for (int k = 0; k < 3; ++k) {
+        lo3[k] = s->lo[k];
+        hi3[k] = s->hi[k];
+    }
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_morton_order(const cwbvh_ray* d_rays, int64_t n, const float* lo3, const float* hi3, int32_t* d_order,
+                           void* stream) {
+    CW_CHECK_ARG(n >= 0 && n <= 0x7FFFFFFF, "ray count must be in [0, 2^31)");
+    CW_CHECK_ARG(n == 0 || (d_rays && d_order && lo3 && hi3), "NULL argument");
+    if (n == 0) return CWBVH_OK;
+    hipStream_t st = (hipStream_t)stream;
+    unsigned *k0 = nullptr, *k1 = nullptr;
+    int32_t* v0 = nullptr;
+    void* tmp = nullptr;
+    size_t tmp_bytes = 0;
+    int32_t rc = CWBVH_OK;
+    hipError_t e = hipMalloc(&k0, n * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&k1, n * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMalloc(&v0, n * sizeof(int32_t));
+    if (e == hipSuccess)
+        e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, v0, d_order, (int)n, 0, 30, st);
+    if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16));
+    if (e == hipSuccess) {
+        const int block = 256;
+        hipLaunchKernelGGL(morton_keys_single, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0, st,
+                           (const Ray4*)d_rays, (int)n, make_float3(lo3[0], lo3[1], lo3[2]), scale_of(lo3, hi3), k0,
+                           v0);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, v0, d_order, (int)n, 0, 30, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = hip_fail(e, "cwbvh_morton_order");
+    if (k0) (void)hipFree(k0);
+    if (k1) (void)hipFree(k1);
+    if (v0) (void)hipFree(v0);
+    if (tmp) (void)hipFree(tmp);
+    return rc;
+}
+
+int32_t cwbvh_batch_create(const cwbvh_batch_item* items, int32_t count, uint32_t flags,
+                           const cwbvh_trace_config* cfg, int32_t morton, cwbvh_batch** out) {
+    CW_CHECK_ARG(out != nullptr, "out is NULL");
+    CW_CHECK_ARG(count >= 0 && (count == 0 || items), "bad item array");
+    CW_CHECK_ARG(count < (1 << 24), "too many items");
+    CW_CHECK_ARG((flags & ~(uint32_t)(CWBVH_TRACE_ANY | CWBVH_TRACE_PRIM_ID)) == 0, "unsupported flags for a batch");
+    cwbvh_trace_config c;
+    if (cfg) c = *cfg;
+    else cwbvh_trace_config_default(CWBVH_KIND_CWBVH, &c);
+    int32_t rc = validate_config(CWBVH_KIND_CWBVH, c);
+    if (rc) return rc;
+    auto* b = new cwbvh_batch();
+    b->flags = flags;
+    b->cfg = c;
+    b->morton = morton ? 1 : 0;
+    uint64_t chunks = 0;
+    std::vector<MortonJob> mj;
+    for (int32_t i = 0; i < count; ++i) {
+        const cwbvh_batch_item& it = items[i];
+        if (!it.scene || it.scene->kind != CWBVH_KIND_CWBVH || it.n < 0 || it.n > 0x7FFFFFFF ||
+            (it.n > 0 && (!it.d_rays || !it.d_hits)) ||
+            ((flags & CWBVH_TRACE_PRIM_ID) && !it.scene->prim_idx)) {
+            delete b;
+            set_error("batch item " + std::to_string(i) + " is invalid (needs a CWBVH scene, rays and hits)");
+            return CWBVH_ERR_INVALID_ARGUMENT;
+        }
+        if (it.n == 0) continue;
+        BatchJob J{};
+        J.nodes = it.scene->nodes;
+        J.woop = it.scene->woop;
+        J.rays = it.d_rays;
+        J.order = nullptr;
+        J.hits = it.d_hits;
+        J.prim_idx = it.scene->prim_idx;
+        J.n = (int32_t)it.n;
+        J.node_stride = it.scene->node_stride;
+        J.chunk_begin = (uint32_t)chunks;
+        chunks += ((uint64_t)it.n + c.fetch_chunk - 1) / c.fetch_chunk;
+        if (chunks > 0xFFFFFFF0ull) {
+            delete b;
+            set_error("batch too large for 32-bit chunk indices");
+            return CWBVH_ERR_INVALID_ARGUMENT;
+        }
+        MortonJob m{};
+        m.rays = (const Ray4*)it.d_rays;
+        m.first = b->total_rays;
+        m.n = (int)it.n;
+        float3 sc = scale_of(it.scene->lo, it.scene->hi);
+        for (int k = 0; k < 3; ++k) m.lo[k] = it.scene->lo[k];
+        m.scale[0] = sc.x;
+        m.scale[1] = sc.y;
+        m.scale[2] = sc.z;
+        mj.push_back(m);
+        b->ray_first.push_back(b->total_rays);
+        b->total_rays += it.n;
+        b->jobs.push_back(J);
+    }
+    b->total_chunks = (uint32_t)chunks;
+    if (b->morton && b->total_rays > 0x7FFFFFFF) {
+        delete b;
+        set_error("Morton pre-pass supports < 2^31 rays per batch");
+        return CWBVH_ERR_INVALID_ARGUMENT;
+    }
+    hipError_t e = hipSuccess;
+    const size_t jb = std::max<size_t>(1, b->jobs.size()) * sizeof(BatchJob);
+    e = hipMalloc(&b->d_jobs, jb);
+    if (e == hipSuccess) e = hipMalloc(&b->counters, kWorkWords * sizeof(unsigned));
+    if (e == hipSuccess) e = hipMemset(b->counters, 0, kWorkWords * sizeof(unsigned));
+    b->bytes = (int64_t)(jb + kWorkWords * sizeof(unsigned));
+    if (e == hipSuccess && b->morton && b->total_rays > 0) {
+        const size_t n = (size_t)b->total_rays;
+        b->key_bits = 30 + bits_for((int)b->jobs.size());
+        e = hipMalloc(&b->d_mjobs, mj.size() * sizeof(MortonJob));
+        if (e == hipSuccess) e = hipMemcpy(b->d_mjobs, mj.data(), mj.size() * sizeof(MortonJob), hipMemcpyHostToDevice);
+        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+            e = hipMalloc(&b->d_keys[k], n * sizeof(unsigned long long));
+            if (e == hipSuccess) e = hipMalloc(&b->d_vals[k], n * sizeof(int32_t));
+        }
+        if (e == hipSuccess)
+            e = hipcub::DeviceRadixSort::SortPairs(nullptr, b->sort_tmp_bytes, b->d_keys[0], b->d_keys[1], b->d_vals[0],
+                                                   b->d_vals[1], (int)n, 0, b->key_bits, (hipStream_t)0);
+        if (e == hipSuccess) e = hipMalloc(&b->d_sort_tmp, std::max<size_t>(b->sort_tmp_bytes, 16));
+        b->bytes += (int64_t)(mj.size() * sizeof(MortonJob) + n * 24 + b->sort_tmp_bytes);
+        // job j reads its rays in the order stored in its slice of the sorted values
+        for (size_t j = 0; j < b->jobs.size(); ++j) b->jobs[j].order = b->d_vals[1] + b->ray_first[j];
+    }
+    if (e == hipSuccess && !b->jobs.empty())
+        e = hipMemcpy(b->d_jobs, b->jobs.data(), b->jobs.size() * sizeof(BatchJob), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        cwbvh_batch_destroy(b);
+        return hip_fail(e, "cwbvh_batch_create");
+    }
+    *out = b;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_batch_launch(cwbvh_batch* b, void* stream) {
+    CW_CHECK_ARG(b != nullptr, "batch is NULL");
+    hipStream_t st = (hipStream_t)stream;
+    int32_t rc = batch_sort(b, st);
+    if (rc) return rc;
+    return batch_trace(b, st);
+}
+
+int32_t cwbvh_batch_capture(cwbvh_batch* b, void* stream) {
+    CW_CHECK_ARG(b != nullptr, "batch is NULL");
+    CW_CHECK_ARG(stream != nullptr, "graph capture needs a non-null stream");
+    hipStream_t st = (hipStream_t)stream;
+    if (b->exec) {
+        (void)hipGraphExecDestroy(b->exec);
+        b->exec = nullptr;
+    }
+    if (b->graph) {
+        (void)hipGraphDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    // the persistent kernel resets its own fetch counters when its last wave
+    // exits, so the graph holds the traversal launch alone (no memset node)
+    CW_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int32_t rc = batch_trace(b, st);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    b->graph = g;
+    CW_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_batch_replay(cwbvh_batch* b, void* stream) {
+    CW_CHECK_ARG(b != nullptr, "batch is NULL");
+    if (!b->exec) {
+        set_error("batch has no captured graph");
+        return CWBVH_ERR_INVALID_ARGUMENT;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    int32_t rc = batch_sort(b, st);
+    if (rc) return rc;
+    CW_HIP(hipGraphLaunch(b->exec, st));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_batch_info(const cwbvh_batch* b, int64_t* total_rays, int32_t* num_jobs, int64_t* device_bytes) {
+    CW_CHECK_ARG(b != nullptr, "batch is NULL");
+    if (total_rays) *total_rays = b->total_rays;
+    if (num_jobs) *num_jobs = (int32_t)b->jobs.size();
+    if (device_bytes) *device_bytes = b->bytes;
+    return CWBVH_OK;
+}
+
+void cwbvh_batch_destroy(cwbvh_batch* b) {
+    if (!b) return;
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    if (b->d_jobs) (void)hipFree(b->d_jobs);
+    if (b->counters) (void)hipFree(b->counters);
+    if (b->d_mjobs) (void)hipFree(b->d_mjobs);
+    for (int k = 0; k < 2; ++k) {
+        if (b->d_keys[k]) (void)hipFree(b->d_keys[k]);
+        if (b->d_vals[k]) (void)hipFree(b->d_vals[k]);
+    }
+    if (b->d_sort_tmp) (void)hipFree(b->d_sort_tmp);
+    delete b;
+}
+
+int32_t cwbvh_autotune_batch(const cwbvh_batch_item* items, int32_t count, uint32_t flags,
+                             const cwbvh_tune_space* space, int32_t reps, cwbvh_tune_record* records,
+                             int32_t capacity, int32_t* n_records, cwbvh_tune_record* best, void* stream) {
+    CW_CHECK_ARG(space != nullptr && best != nullptr, "NULL argument");
+    CW_CHECK_ARG(space->n_block >= 1 && space->n_block <= 4 && space->n_lds >= 1 && space->n_lds <= 4 &&
+                     space->n_morton >= 1 && space->n_morton <= 2 && space->n_prefetch >= 1 && space->n_prefetch <= 2,
+                 "bad tuning space");
+    CW_CHECK_ARG(reps >= 1 && reps <= 1000, "reps must be in [1, 1000]");
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    CW_HIP(hipEventCreate(&e0));
+    CW_HIP(hipEventCreate(&e1));
+    int32_t n = 0;
+    best->ms = INFINITY;
+    int32_t rc = CWBVH_OK;
+    for (int ib = 0; ib < space->n_block && rc == CWBVH_OK; ++ib)
+        for (int il = 0; il < space->n_lds && rc == CWBVH_OK; ++il)
+            for (int im = 0; im < space->n_morton && rc == CWBVH_OK; ++im)
+                for (int ip = 0; ip < space->n_prefetch && rc == CWBVH_OK; ++ip) {
+                    cwbvh_trace_config c;
+                    cwbvh_trace_config_default(CWBVH_KIND_CWBVH, &c);
+                    c.block_size = space->block_sizes[ib];
+                    c.lds_depth = space->lds_depths[il];
+                    c.prefetch = space->prefetch[ip];
+                    cwbvh_batch* b = nullptr;
+                    rc = cwbvh_batch_create(items, count, flags, &c, space->morton[im], &b);
+                    if (rc) break;
+                    std::vector<float> t;
+                    rc = cwbvh_batch_launch(b, st);  // warm-up
+                    for (int r = 0; r < reps && rc == CWBVH_OK; ++r) {
+                        (void)hipEventRecord(e0, st);
+                        rc = cwbvh_batch_launch(b, st);
+                        (void)hipEventRecord(e1, st);
+                        hipError_t he = hipEventSynchronize(e1);
+                        if (he != hipSuccess) rc = hip_fail(he, "autotune sync");
+                        float ms = 0.f;
+                        (void)hipEventElapsedTime(&ms, e0, e1);
+                        t.push_back(ms);
+                    }
+                    cwbvh_batch_destroy(b);
+                    if (rc) break;
+                    std::sort(t.begin(), t.end());
+                    cwbvh_tune_record rec{};
+                    rec.cfg = c;
+                    rec.morton = space->morton[im];
+                    rec.ms = t[t.size() / 2];
+                    if (records && n < capacity) records[n] = rec;
+                    ++n;
+                    if (rec.ms < best->ms) *best = rec;
+                }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (n_records) *n_records = n;
+    return rc;
+}
+
+}  // extern "C"

@@ -20,6 +20,8 @@ struct cwbvh_scene {
     bool owning = false;
     int64_t bytes = 0;
     int32_t device = 0;
+    float lo[3] = {0.f, 0.f, 0.f};  // scene bounds (Morton ordering); unit cube if unknown
+    float hi[3] = {1.f, 1.f, 1.f};
 };
 
 namespace cw {
@@ -37,10 +39,13 @@ int32_t hip_fail(hipError_t e, const char* what);
 // The kernel resets the fetch counters and the exit count when its last wave
 // exits, so launches on one stream reuse the same counters (and a captured
 // graph replays cleanly).  The overflow flag stays set until read.
+// Each counter sits on its own 256-byte line so that the slices' atomics are
+// served by different memory channels instead of serializing on one line.
 constexpr int kWorkSlices = 8;
-constexpr int kWorkExited = 8;
-constexpr int kWorkOverflow = 9;
-constexpr int kWorkWords = 16;
+constexpr int kWorkStride = 64;                       // uint32 per counter line
+constexpr int kWorkExited = kWorkSlices * kWorkStride;
+constexpr int kWorkOverflow = kWorkExited + kWorkStride;
+constexpr int kWorkWords = kWorkOverflow + kWorkStride;
 int32_t stream_counters(hipStream_t stream, unsigned** out);
 
 struct TraceLaunch {

@@ -1,0 +1,523 @@
+// Serving loop: resident scenes, HBM-budget admission, continuous batching.
+//
+// No counterpart in the reference (a single-shot demo, src/main.cpp).  One
+// scheduler thread owns the device side:
+//   * a device arena of `memory_budget` bytes holds the rays and hits of every
+//     admitted request (48 B per ray); requests wait in FIFO order until the
+//     arena has room, so the budget bounds the HBM held by in-flight work;
+//   * up to `max_inflight` batch slots, each with its own stream, job table and
+//     fetch counters: whenever a slot is free, every pending request that fits
+//     the arena (up to the batch limits) is copied in and traversed by ONE
+//     persistent multi-scene launch, so requests that arrive while earlier
+//     batches are on the GPU join the next batch and overlap with them;
+//   * completion is detected by polling each slot's event; hits are copied
+//     back on the slot's stream before the event, arena space is released and
+//     waiters are woken.
+// Every ray's result is independent of which batch, slot or wave traversed it
+// (deterministic tie rule), so a request returns the same hits alone and in
+// the mix.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <random>
+#include <thread>
+#include <unordered_map>
+
+#include "internal.h"
+
+using namespace cw;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double now_s() { return std::chrono::duration<double>(Clock::now().time_since_epoch()).count(); }
+
+constexpr int64_t kAlign = 256;
+
+int64_t align_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+// First-fit allocator over [0, size) with coalescing.
+class Arena {
+public:
+    explicit Arena(int64_t size = 0) { reset(size); }
+    void reset(int64_t size) {
+        free_.clear();
+        if (size > 0) free_[0] = size;
+        used_ = 0;
+    }
+    int64_t alloc(int64_t bytes) {
+        bytes = align_up(std::max<int64_t>(bytes, 1));
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= bytes) {
+                int64_t off = it->first, rest = it->second - bytes;
+                free_.erase(it);
+                if (rest > 0) free_[off + bytes] = rest;
+                used_ += bytes;
+                return off;
+            }
+        }
+        return -1;
+    }
+    void release(int64_t off, int64_t bytes) {
+        bytes = align_up(std::max<int64_t>(bytes, 1));
+        used_ -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    int64_t used() const { return used_; }
+
+private:
+    std::map<int64_t, int64_t> free_;
+    int64_t used_ = 0;
+};
+
+struct Request {
+    uint64_t id = 0;
+    int32_t scene = 0;
+    const cwbvh_ray* host_rays = nullptr;
+    cwbvh_hit* host_hits = nullptr;
+    int64_t n = 0;
+    int64_t off = -1;        // arena offset (rays, then hits)
+    double t_submit = 0, t_done = 0;
+    bool done = false;
+    int32_t status = CWBVH_OK;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    BatchJob* d_jobs = nullptr;
+    BatchJob* h_jobs = nullptr;   // pinned
+    unsigned* counters = nullptr;
+    std::vector<Request*> reqs;
+    bool busy = false;
+};
+
+}  // namespace
+
+struct cwbvh_server {
+    std::vector<const cwbvh_scene*> scenes;
+    cwbvh_server_config cfg{};
+    char* d_arena = nullptr;
+    Arena arena;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<Request*> pending;
+    std::unordered_map<uint64_t, Request*> requests;
+    uint64_t next_id = 1;
+    int64_t outstanding = 0;
+    bool stop = false;
+    int32_t worker_status = CWBVH_OK;
+    std::string worker_error;
+    std::thread worker;
+    // stats (guarded by mu)
+    cwbvh_server_stats stats{};
+    int inflight = 0;
+    double busy_since = 0;
+
+    int32_t launch(Slot& sl, const std::vector<Request*>& batch);
+    void complete(Slot& sl);
+    void loop();
+};
+
+int32_t cwbvh_server::launch(Slot& sl, const std::vector<Request*>& batch) {
+    const int32_t chunk = cfg.trace.fetch_chunk;
+    uint32_t chunks = 0;
+    for (size_t k = 0; k < batch.size(); ++k) {
+        const Request* r = batch[k];
+        char* base = d_arena + r->off;
+        CW_HIP(hipMemcpyAsync(base, r->host_rays, (size_t)r->n * sizeof(cwbvh_ray), hipMemcpyHostToDevice, sl.stream));
+        const cwbvh_scene* sc = scenes[r->scene];
+        BatchJob& J = sl.h_jobs[k];
+        J.nodes = sc->nodes;
+        J.woop = sc->woop;
+        J.rays = base;
+        J.order = nullptr;
+        J.hits = base + align_up(r->n * (int64_t)sizeof(cwbvh_ray));
+        J.prim_idx = sc->prim_idx;
+        J.n = (int32_t)r->n;
+        J.node_stride = sc->node_stride;
+        J.chunk_begin = chunks;
+        J.pad = 0;
+        chunks += (uint32_t)((r->n + chunk - 1) / chunk);
+    }
+    if (cfg.serial) {
+        const Request* r = batch[0];
+        TraceLaunch L{};
+        L.scene = scenes[r->scene];
+        L.rays = sl.h_jobs[0].rays;
+        L.order = nullptr;
+        L.hits = sl.h_jobs[0].hits;
+        L.n = r->n;
+        L.flags = cfg.flags;
+        L.cfg = cfg.trace;
+        L.counters = sl.counters;
+        L.stream = sl.stream;
+        int32_t rc = launch_cwbvh(L);
+        if (rc) return rc;
+    } else {
+        CW_HIP(hipMemcpyAsync(sl.d_jobs, sl.h_jobs, batch.size() * sizeof(BatchJob), hipMemcpyHostToDevice, sl.stream));
+        BatchLaunch L{};
+        L.d_jobs = sl.d_jobs;
+        L.num_jobs = (int32_t)batch.size();
+        L.total_chunks = chunks;
+        L.flags = cfg.flags;
+        L.cfg = cfg.trace;
+        L.counters = sl.counters;
+        L.stream = sl.stream;
+        int32_t rc = launch_cwbvh_batch(L);
+        if (rc) return rc;
+    }
+    for (size_t k = 0; k < batch.size(); ++k) {
+        const Request* r = batch[k];
+        CW_HIP(hipMemcpyAsync(r->host_hits, sl.h_jobs[k].hits, (size_t)r->n * sizeof(cwbvh_hit), hipMemcpyDeviceToHost,
+                              sl.stream));
+    }
+    CW_HIP(hipEventRecord(sl.done, sl.stream));
+    sl.reqs = batch;
+    sl.busy = true;
+    return CWBVH_OK;
+}
+
+void cwbvh_server::complete(Slot& sl) {
+    const double t = now_s();
+    std::lock_guard<std::mutex> g(mu);
+    for (Request* r : sl.reqs) {
+        r->t_done = t;
+        r->done = true;
+        arena.release(r->off, align_up(r->n * (int64_t)sizeof(cwbvh_ray)) + r->n * (int64_t)sizeof(cwbvh_hit));
+        stats.completed++;
+        stats.rays += r->n;
+        --outstanding;
+    }
+    sl.reqs.clear();
+    sl.busy = false;
+    if (--inflight == 0) stats.busy_s += t - busy_since;
+    cv_done.notify_all();
+}
+
+void cwbvh_server::loop() {
+    std::vector<Request*> batch;
+    while (true) {
+        bool progress = false;
+        for (Slot& sl : slots) {
+            if (!sl.busy) continue;
+            hipError_t q = hipEventQuery(sl.done);
+            if (q == hipSuccess) {
+                complete(sl);
+                progress = true;
+            } else if (q != hipErrorNotReady) {
+                std::lock_guard<std::mutex> g(mu);
+                worker_status = hip_fail(q, "serving batch");
+                worker_error = get_error();
+                for (Request* r : sl.reqs) {
+                    r->status = worker_status;
+                    r->done = true;
+                    --outstanding;
+                }
+                sl.reqs.clear();
+                sl.busy = false;
+                cv_done.notify_all();
+            }
+        }
+        Slot* free_slot = nullptr;
+        for (Slot& sl : slots)
+            if (!sl.busy) {
+                free_slot = &sl;
+                break;
+            }
+        batch.clear();
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            if (stop && pending.empty() && inflight == 0) break;
+            if (free_slot && worker_status == CWBVH_OK) {
+                const int32_t max_req = cfg.serial ? 1 : cfg.max_batch_requests;
+                int64_t rays = 0;
+                while (!pending.empty() && (int32_t)batch.size() < max_req) {
+                    Request* r = pending.front();
+                    if (!batch.empty() && rays + r->n > cfg.max_batch_rays) break;
+                    int64_t off = arena.alloc(align_up(r->n * (int64_t)sizeof(cwbvh_ray)) + r->n * (int64_t)sizeof(cwbvh_hit));
+                    if (off < 0) break;  // FIFO: wait for completions to free HBM
+                    r->off = off;
+                    batch.push_back(r);
+                    pending.pop_front();
+                    rays += r->n;
+                }
+                if (!batch.empty()) {
+                    stats.peak_bytes = std::max(stats.peak_bytes, arena.used());
+                    if (inflight++ == 0) busy_since = now_s();
+                    stats.max_inflight_seen = std::max(stats.max_inflight_seen, inflight);
+                    stats.max_batch_requests_seen = std::max(stats.max_batch_requests_seen, (int32_t)batch.size());
+                    stats.batches++;
+                }
+            }
+            if (batch.empty() && !progress) {
+                // nothing to launch: sleep until a submission or a short poll interval
+                cv_work.wait_for(lk, std::chrono::microseconds(inflight ? 20 : 2000));
+                continue;
+            }
+        }
+        if (!batch.empty()) {
+            int32_t rc = launch(*free_slot, batch);
+            if (rc) {
+                std::lock_guard<std::mutex> g(mu);
+                worker_status = rc;
+                worker_error = get_error();
+                for (Request* r : batch) {
+                    r->status = rc;
+                    r->done = true;
+                    arena.release(r->off, align_up(r->n * (int64_t)sizeof(cwbvh_ray)) + r->n * (int64_t)sizeof(cwbvh_hit));
+                    --outstanding;
+                }
+                --inflight;
+                cv_done.notify_all();
+            }
+        }
+    }
+}
+
+extern "C" {
+
+int32_t cwbvh_host_alloc(void** ptr, int64_t bytes) {
+    CW_CHECK_ARG(ptr != nullptr && bytes >= 0, "bad arguments");
+    *ptr = nullptr;
+    if (bytes == 0) return CWBVH_OK;
+    CW_HIP(hipHostMalloc(ptr, (size_t)bytes, hipHostMallocDefault));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_host_free(void* ptr) {
+    if (ptr) CW_HIP(hipHostFree(ptr));
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_server_config_default(cwbvh_server_config* c) {
+    CW_CHECK_ARG(c != nullptr, "cfg is NULL");
+    std::memset(c, 0, sizeof(*c));
+    c->memory_budget = (int64_t)8 << 30;
+    c->max_inflight = 3;
+    c->max_batch_requests = 64;
+    c->max_batch_rays = (int64_t)16 << 20;
+    c->serial = 0;
+    c->flags = CWBVH_TRACE_CLOSEST;
+    return cwbvh_trace_config_default(CWBVH_KIND_CWBVH, &c->trace);
+}
+
+int32_t cwbvh_server_create(const cwbvh_scene* const* scenes, int32_t num_scenes, const cwbvh_server_config* cfg,
+                            cwbvh_server** out) {
+    CW_CHECK_ARG(out != nullptr && scenes != nullptr && num_scenes >= 1, "bad arguments");
+    cwbvh_server_config c;
+    if (cfg) c = *cfg;
+    else cwbvh_server_config_default(&c);
+    CW_CHECK_ARG(c.memory_budget >= 4096 && c.max_inflight >= 1 && c.max_inflight <= 16 &&
+                     c.max_batch_requests >= 1 && c.max_batch_requests <= 4096 && c.max_batch_rays >= 1,
+                 "invalid server configuration");
+    CW_CHECK_ARG((c.flags & ~(uint32_t)(CWBVH_TRACE_ANY | CWBVH_TRACE_PRIM_ID)) == 0, "unsupported flags");
+    int32_t rc = validate_config(CWBVH_KIND_CWBVH, c.trace);
+    if (rc) return rc;
+    for (int32_t i = 0; i < num_scenes; ++i)
+        CW_CHECK_ARG(scenes[i] && scenes[i]->kind == CWBVH_KIND_CWBVH &&
+                         (!(c.flags & CWBVH_TRACE_PRIM_ID) || scenes[i]->prim_idx),
+                     "server scenes must be CWBVH scenes");
+    auto* s = new cwbvh_server();
+    s->scenes.assign(scenes, scenes + num_scenes);
+    s->cfg = c;
+    if (c.serial) s->cfg.max_inflight = 1;
+    hipError_t e = hipMalloc(&s->d_arena, (size_t)c.memory_budget);
+    s->arena.reset(c.memory_budget);
+    s->slots.resize(s->cfg.max_inflight);
+    const int32_t jobs = s->cfg.serial ? 1 : c.max_batch_requests;
+    for (Slot& sl : s->slots) {
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipMalloc(&sl.d_jobs, jobs * sizeof(BatchJob));
+        if (e == hipSuccess) e = hipHostMalloc(&sl.h_jobs, jobs * sizeof(BatchJob), hipHostMallocDefault);
+        if (e == hipSuccess) e = hipMalloc(&sl.counters, kWorkWords * sizeof(unsigned));
+        if (e == hipSuccess) e = hipMemset(sl.counters, 0, kWorkWords * sizeof(unsigned));
+    }
+    if (e != hipSuccess) {
+        cwbvh_server_destroy(s);
+        return hip_fail(e, "cwbvh_server_create");
+    }
+    s->worker = std::thread([s] { s->loop(); });
+    *out = s;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_server_submit(cwbvh_server* s, int32_t scene, const cwbvh_ray* host_rays, int64_t n,
+                            cwbvh_hit* host_hits, uint64_t* request_id) {
+    CW_CHECK_ARG(s != nullptr && request_id != nullptr, "NULL argument");
+    CW_CHECK_ARG(scene >= 0 && scene < (int32_t)s->scenes.size(), "scene index out of range");
+    CW_CHECK_ARG(n >= 0 && n <= 0x7FFFFFFF && (n == 0 || (host_rays && host_hits)), "bad ray buffer");
+    if (align_up(n * (int64_t)sizeof(cwbvh_ray)) + n * (int64_t)sizeof(cwbvh_hit) > s->cfg.memory_budget) {
+        set_error("request of " + std::to_string(n) + " rays exceeds the serving memory budget");
+        return CWBVH_ERR_BUDGET;
+    }
+    auto* r = new Request();
+    r->scene = scene;
+    r->host_rays = host_rays;
+    r->host_hits = host_hits;
+    r->n = n;
+    r->t_submit = now_s();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        if (s->worker_status != CWBVH_OK) {
+            delete r;
+            set_error("server failed: " + s->worker_error);
+            return s->worker_status;
+        }
+        r->id = s->next_id++;
+        s->requests[r->id] = r;
+        if (n == 0) {
+            r->done = true;
+            r->t_done = r->t_submit;
+        } else {
+            s->pending.push_back(r);
+            ++s->outstanding;
+        }
+    }
+    s->cv_work.notify_one();
+    *request_id = r->id;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_server_wait(cwbvh_server* s, uint64_t id, double timeout_s, double* latency_ms) {
+    CW_CHECK_ARG(s != nullptr, "server is NULL");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->requests.find(id);
+    CW_CHECK_ARG(it != s->requests.end(), "unknown request id");
+    Request* r = it->second;
+    auto pred = [r] { return r->done; };
+    if (timeout_s < 0) s->cv_done.wait(lk, pred);
+    else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), pred)) {
+        set_error("timed out");
+        return CWBVH_ERR_CANCELLED;
+    }
+    if (latency_ms) *latency_ms = (r->t_done - r->t_submit) * 1e3;
+    int32_t st = r->status;
+    s->requests.erase(it);
+    delete r;
+    if (st) set_error(s->worker_error);
+    return st;
+}
+
+int32_t cwbvh_server_drain(cwbvh_server* s) {
+    CW_CHECK_ARG(s != nullptr, "server is NULL");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [s] { return s->outstanding == 0; });
+    return s->worker_status;
+}
+
+int32_t cwbvh_server_get_stats(const cwbvh_server* s, cwbvh_server_stats* out) {
+    CW_CHECK_ARG(s != nullptr && out != nullptr, "NULL argument");
+    auto* m = const_cast<cwbvh_server*>(s);
+    std::lock_guard<std::mutex> g(m->mu);
+    *out = m->stats;
+    return CWBVH_OK;
+}
+
+void cwbvh_server_destroy(cwbvh_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->worker.joinable()) s->worker.join();
+    for (auto& kv : s->requests) delete kv.second;
+    for (Slot& sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.d_jobs) (void)hipFree(sl.d_jobs);
+        if (sl.h_jobs) (void)hipHostFree(sl.h_jobs);
+        if (sl.counters) (void)hipFree(sl.counters);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    }
+    if (s->d_arena) (void)hipFree(s->d_arena);
+    delete s;
+}
+
+int32_t cwbvh_server_run_poisson(cwbvh_server* s, const cwbvh_poisson_spec* spec, const cwbvh_ray* const* ray_pools,
+                                 const int64_t* pool_sizes, cwbvh_hit* hit_pool, int64_t hit_pool_size,
+                                 double* latencies_ms, cwbvh_poisson_result* out) {
+    CW_CHECK_ARG(s && spec && ray_pools && pool_sizes && hit_pool && out, "NULL argument");
+    CW_CHECK_ARG(spec->rate > 0 && spec->num_requests >= 1 && spec->min_rays >= 1 && spec->max_rays >= spec->min_rays,
+                 "bad Poisson spec");
+    const int32_t ns = (int32_t)s->scenes.size();
+    for (int32_t k = 0; k < ns; ++k) CW_CHECK_ARG(ray_pools[k] && pool_sizes[k] >= 1, "empty ray pool");
+    CW_CHECK_ARG(hit_pool_size >= spec->max_rays, "hit pool smaller than the largest request");
+    std::mt19937_64 rng(spec->seed);
+    std::exponential_distribution<double> gap(spec->rate);
+    std::uniform_real_distribution<double> u01(0.0, 1.0);
+    std::vector<uint64_t> ids(spec->num_requests);
+    std::vector<int64_t> sizes(spec->num_requests);
+    int64_t ring = 0;
+    const double lmin = std::log((double)spec->min_rays), lmax = std::log((double)spec->max_rays);
+    const double t0 = now_s();
+    double t_next = t0;
+    int32_t rc = CWBVH_OK;
+    for (int32_t k = 0; k < spec->num_requests && rc == CWBVH_OK; ++k) {
+        t_next += gap(rng);
+        const int32_t sc = (int32_t)(u01(rng) * ns) % ns;
+        int64_t n = (int64_t)std::exp(lmin + (lmax - lmin) * u01(rng));
+        n = std::min(std::max<int64_t>(n, spec->min_rays), std::min<int64_t>(spec->max_rays, pool_sizes[sc]));
+        const int64_t off = pool_sizes[sc] > n ? (int64_t)(u01(rng) * (double)(pool_sizes[sc] - n)) : 0;
+        if (ring + n > hit_pool_size) ring = 0;
+        while (now_s() < t_next) {
+            // arrival clock: sleep when far ahead, spin for the last tens of microseconds
+            const double ahead = t_next - now_s();
+            if (ahead > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(ahead - 1e-4));
+        }
+        rc = cwbvh_server_submit(s, sc, ray_pools[sc] + off, n, hit_pool + ring, &ids[k]);
+        ring += n;
+        sizes[k] = n;
+    }
+    if (rc) return rc;
+    std::vector<double> lat(spec->num_requests);
+    double t_last = t0;
+    for (int32_t k = 0; k < spec->num_requests; ++k) {
+        rc = cwbvh_server_wait(s, ids[k], -1.0, &lat[k]);
+        if (rc) return rc;
+    }
+    t_last = now_s();
+    int64_t rays = 0;
+    for (int64_t v : sizes) rays += v;
+    if (latencies_ms) std::copy(lat.begin(), lat.end(), latencies_ms);
+    std::vector<double> sorted = lat;
+    std::sort(sorted.begin(), sorted.end());
+    auto pct = [&](double p) {
+        const size_t i = (size_t)std::min<double>((double)sorted.size() - 1, std::ceil(p * sorted.size()) - 1);
+        return sorted[i];
+    };
+    double mean = 0;
+    for (double v : lat) mean += v;
+    out->wall_s = t_last - t0;
+    out->p50_ms = pct(0.50);
+    out->p99_ms = pct(0.99);
+    out->mean_ms = mean / lat.size();
+    out->max_ms = sorted.back();
+    out->rays = rays;
+    out->mrays_per_s = rays / out->wall_s / 1e6;
+    return CWBVH_OK;
+}
+
+}  // extern "C"

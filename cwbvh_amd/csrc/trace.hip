@@ -39,6 +39,7 @@ namespace {
 
 constexpr int kMaxBlock = 256;
 constexpr int kWorkSlices = cw::kWorkSlices;
+constexpr int kWorkStride = cw::kWorkStride;
 constexpr int kWorkExited = cw::kWorkExited;
 constexpr int kWorkOverflow = cw::kWorkOverflow;
 
@@ -54,6 +55,7 @@ struct CwbvhParams {
     unsigned long long* stats;      // {nodes, triangles}
     int n;                          // rays (single) or total chunks (multi)
     int slices;                     // single scene: contiguous ray ranges, one per XCD
+    int persistent;                 // 0: one ray per thread, no refill
     int num_jobs;
     int stride;                     // float4 per node
     int lds_depth;
@@ -144,7 +146,21 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
             if (pool_next >= pool_end) {
                 if (exhausted) break;
                 const int leader = __ffsll((long long)need) - 1;
-                if (MULTI) {
+                if (!MULTI && !P.persistent) {
+                    // one ray per thread: the only "pool" is this wave's own 64 indices
+                    if (slice_k > 0) {
+                        exhausted = true;
+                        break;
+                    }
+                    slice_k = 1;
+                    const unsigned first = (blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
+                    pool_next = min(first, (unsigned)P.n);
+                    pool_end = min(first + 64u, (unsigned)P.n);
+                    if (pool_next >= pool_end) {
+                        exhausted = true;
+                        break;
+                    }
+                } else if (MULTI) {
                     unsigned base = 0;
                     if (lane == leader) base = atomicAdd(&P.work[0], 1u);
                     base = __shfl(base, leader);
@@ -170,7 +186,7 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
                         const unsigned lo = (unsigned)(((unsigned long long)P.n * (unsigned)sl) / (unsigned)P.slices);
                         const unsigned hi = (unsigned)(((unsigned long long)P.n * (unsigned)(sl + 1)) / (unsigned)P.slices);
                         unsigned base = 0;
-                        if (lane == leader) base = atomicAdd(&P.work[sl], (unsigned)P.chunk);
+                        if (lane == leader) base = atomicAdd(&P.work[sl * kWorkStride], (unsigned)P.chunk);
                         base = __shfl(base, leader);
                         if (base < hi - lo) {
                             pool_next = lo + base;
@@ -373,7 +389,7 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
     if (lane == __ffsll((long long)alive) - 1) {
         const unsigned prev = atomicAdd(&P.work[kWorkExited], 1u);
         if (prev == (unsigned)P.total_waves - 1u) {
-            for (int k = 0; k < kWorkSlices; ++k) atomicExch(&P.work[k], 0u);
+            for (int k = 0; k < kWorkSlices; ++k) atomicExch(&P.work[k * kWorkStride], 0u);
             atomicExch(&P.work[kWorkExited], 0u);
         }
     }
@@ -531,10 +547,23 @@ int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_
     const int block = cfg.block_size;
     const size_t lds = (size_t)block * (size_t)cfg.lds_depth * sizeof(unsigned long long);
     auto kernel = cwbvh_trace<ANY, STATS, MULTI, PREFETCH>;
-    int grid = cfg.grid_blocks;
-    if (grid <= 0) grid = resident_blocks(kernel, block, lds) * cw::device_compute_units();
-    p.total_waves = grid * (block / 64);
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(block), lds, stream, p);
+    long long grid = cfg.grid_blocks;
+    if (!MULTI && !p.persistent) {
+        grid = ((long long)p.n + block - 1) / block;
+        p.idle_budget = 0x7FFFFFFF;  // nothing to refill from
+    } else if (grid <= 0) {
+        grid = (long long)resident_blocks(kernel, block, lds) * cw::device_compute_units();
+    }
+    p.total_waves = (int)(grid * (block / 64));
+    if (p.chunk <= 0) {
+        // automatic: about 16 claims per wave over the launch, 64..1024 rays each
+        const long long rays = MULTI ? 0 : p.n;
+        long long c = rays / ((long long)p.total_waves * 16);
+        if (MULTI) c = 256;
+        c = std::min<long long>(1024, std::max<long long>(64, (c + 63) / 64 * 64));
+        p.chunk = (int)c;
+    }
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(block), lds, stream, p);
     CW_HIP(hipGetLastError());
     return CWBVH_OK;
 }
@@ -585,7 +614,8 @@ int32_t validate_config(int32_t kind, const cwbvh_trace_config& c) {
         return CWBVH_ERR_INVALID_ARGUMENT;
     }
     if (kind == CWBVH_KIND_CWBVH) {
-        if (c.fetch_chunk < 1 || c.fetch_chunk > (1 << 20) || c.grid_blocks < 0 || c.refill_budget < 1 ||
+        if (c.fetch_chunk < 0 || c.fetch_chunk > (1 << 20) || c.grid_blocks < 0 || c.refill_budget < 1 ||
+            c.persistent < 0 || c.persistent > 1 ||
             c.refill_idle < 0 || c.refill_idle > 64 || c.postpone_pct < 0 || c.postpone_pct > 100 ||
             c.prefetch < 0 || c.prefetch > 1 || c.work_slices < 1 || c.work_slices > kWorkSlices) {
             set_error("invalid CWBVH trace configuration");
@@ -617,6 +647,7 @@ int32_t launch_cwbvh(const TraceLaunch& L) {
     p.postpone_pct = L.cfg.postpone_pct;
     p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
     p.slices = L.cfg.work_slices;
+    p.persistent = L.cfg.persistent;
     return dispatch_cwbvh<false>(p, L.flags, L.cfg, L.stream);
 }
 
@@ -634,6 +665,7 @@ int32_t launch_cwbvh_batch(const BatchLaunch& L) {
     p.postpone_pct = L.cfg.postpone_pct;
     p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
     p.slices = 1;
+    p.persistent = 1;
     return dispatch_cwbvh<true>(p, L.flags, L.cfg, L.stream);
 }
 

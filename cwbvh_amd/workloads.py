@@ -143,3 +143,25 @@ def c3(num_triangles: int = 20_000_000, num_rays: int = 64 * 1024 * 1024, seed: 
     rays = rays_sphere_from_surface(m, num_rays, 1e-4 * diag, 0.0, np.inf, seed=seed + 1)
     return Workload("C3", m, rays, diag, seed,
                     f"cluster soup {m.num_triangles} tris / 4096 clusters, {len(rays)} diffuse rays, tmax=inf")
+
+
+# ------------------------------------------------------------------- C4
+
+
+def c4_plan(num_scenes: int = 32, total_tris: int = 100_000_000, seed: int = SEEDS["C4"]) -> list[dict]:
+    """Resident scene set: heightfields, icosphere lattices and clustered soups
+    in rotation, sizes log-uniform around total_tris / num_scenes."""
+    rng = np.random.default_rng(seed)
+    raw = np.exp(rng.uniform(np.log(0.25), np.log(4.0), size=num_scenes))
+    sizes = (raw / raw.sum() * total_tris).astype(np.int64)
+    kinds = ("heightfield", "icospheres", "soup")
+    return [{"index": i, "kind": kinds[i % 3], "target_tris": int(max(1000, sizes[i])), "seed": seed + 104729 * (i + 1)}
+            for i in range(num_scenes)]
+
+
+def c4_mesh(entry: dict) -> Mesh:
+    if entry["kind"] == "heightfield":
+        return heightfield_of(entry["target_tris"], entry["seed"])
+    if entry["kind"] == "icospheres":
+        return icospheres_of(entry["target_tris"], entry["seed"])
+    return Mesh.cluster_soup(entry["target_tris"], 512, 0.005, 0.05, math.log(1e-3), 0.5, seed=entry["seed"])

@@ -258,12 +258,13 @@ typedef struct cwbvh_trace_config {
     int32_t block_size;      /* 64, 128 or 256 threads */
     int32_t lds_depth;       /* stack entries per lane kept in LDS (0..stack capacity) */
     int32_t grid_blocks;     /* persistent CWBVH grid (0 = fill the device) */
-    int32_t fetch_chunk;     /* rays claimed per wave-level atomic (CWBVH) */
+    int32_t fetch_chunk;     /* rays claimed per wave-level atomic (CWBVH; 0 = automatic) */
     int32_t refill_idle;     /* leave the inner loop once idle lanes exceed this ... */
     int32_t refill_budget;   /* ... for this many accumulated lane-iterations */
     int32_t postpone_pct;    /* postpone a triangle group below this % of active lanes (0 = off) */
     int32_t prefetch;        /* 0 or 1: issue the next node's loads before the triangle loop */
     int32_t work_slices;     /* 1..8 contiguous ray ranges, one drained first per XCD (CWBVH) */
+    int32_t persistent;      /* CWBVH: 1 = persistent grid with work fetching, 0 = one ray per thread */
 } cwbvh_trace_config;
 
 int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
@@ -283,6 +284,148 @@ int32_t cwbvh_trace_ordered(const cwbvh_scene* scene, const cwbvh_ray* d_rays,
 /* 1 when a traversal on this stream overflowed its stack since the last
  * call (the flag is cleared); the affected rays may have missed hits. */
 int32_t cwbvh_stack_overflowed(void* stream, int32_t* flag);
+
+/* Scene bounds (lo[3], hi[3]) as stored at creation (wrapped scenes: unit cube). */
+int32_t cwbvh_scene_bounds(const cwbvh_scene* scene, float* lo3, float* hi3);
+
+/* Morton ray order: d_order[i] = index of the i-th ray when rays are sorted by
+ * the 30-bit Morton code of their origin quantized inside [lo, hi].
+ * d_order holds n int32.  Temporary device memory is allocated per call. */
+int32_t cwbvh_morton_order(const cwbvh_ray* d_rays, int64_t n, const float* lo3, const float* hi3,
+                           int32_t* d_order, void* stream);
+
+/* ------------------------------------------- multi-scene batches --- */
+
+typedef struct cwbvh_batch_item {
+    const cwbvh_scene* scene;    /* CWBVH scene (node stride 5 or 8) */
+    const cwbvh_ray* d_rays;
+    cwbvh_hit* d_hits;
+    int64_t n;
+} cwbvh_batch_item;
+
+typedef struct cwbvh_batch cwbvh_batch;
+
+/* Plan one persistent launch that traverses every item: a device job table
+ * (one 64-byte record per item) and private fetch counters.  morton != 0 adds
+ * a per-launch pre-pass that orders every item's rays by origin Morton code
+ * (key = item index : Morton code, one radix sort over all rays). */
+int32_t cwbvh_batch_create(const cwbvh_batch_item* items, int32_t count, uint32_t flags,
+                           const cwbvh_trace_config* cfg, int32_t morton, cwbvh_batch** out);
+/* Morton pre-pass (if enabled) + the persistent traversal, on `stream`. */
+int32_t cwbvh_batch_launch(cwbvh_batch* batch, void* stream);
+/* Capture the traversal launch into a hipGraph (instantiated once); the
+ * Morton pre-pass, when enabled, stays a stream-ordered step in front of it. */
+int32_t cwbvh_batch_capture(cwbvh_batch* batch, void* stream);
+/* Morton pre-pass (if enabled) + replay of the captured graph. */
+int32_t cwbvh_batch_replay(cwbvh_batch* batch, void* stream);
+int32_t cwbvh_batch_info(const cwbvh_batch* batch, int64_t* total_rays, int32_t* num_jobs,
+                         int64_t* device_bytes);
+void cwbvh_batch_destroy(cwbvh_batch* batch);
+
+/* ------------------------------------------------------------ serving --- */
+
+/* Pinned host memory (for request rays / hits that the server copies). */
+int32_t cwbvh_host_alloc(void** ptr, int64_t bytes);
+int32_t cwbvh_host_free(void* ptr);
+
+typedef struct cwbvh_server cwbvh_server;
+
+typedef struct cwbvh_server_config {
+    int64_t memory_budget;        /* device bytes for in-flight ray + hit buffers (48 B / ray) */
+    int32_t max_inflight;         /* batches on the device at once (one stream each), 1..16 */
+    int32_t max_batch_requests;   /* requests per batch launch */
+    int64_t max_batch_rays;       /* rays per batch launch */
+    int32_t serial;               /* 1: one request at a time with a plain per-scene launch */
+    uint32_t flags;               /* CWBVH_TRACE_CLOSEST or CWBVH_TRACE_ANY (+ PRIM_ID) */
+    cwbvh_trace_config trace;     /* kernel configuration */
+} cwbvh_server_config;
+
+int32_t cwbvh_server_config_default(cwbvh_server_config* cfg);
+
+/* Resident scenes (CWBVH, not owned; they must outlive the server). */
+int32_t cwbvh_server_create(const cwbvh_scene* const* scenes, int32_t num_scenes,
+                            const cwbvh_server_config* cfg, cwbvh_server** out);
+/* Queue one request: n host rays against scenes[scene]; the hits are copied
+ * to host_hits when the request completes.  Host buffers should be pinned
+ * (cwbvh_host_alloc) and must stay valid until completion.  Requests larger
+ * than the memory budget fail with CWBVH_ERR_BUDGET. */
+int32_t cwbvh_server_submit(cwbvh_server* s, int32_t scene, const cwbvh_ray* host_rays, int64_t n,
+                            cwbvh_hit* host_hits, uint64_t* request_id);
+/* Wait for a request (timeout_s < 0: no limit); *latency_ms = completion -
+ * submission on the host clock. */
+int32_t cwbvh_server_wait(cwbvh_server* s, uint64_t request_id, double timeout_s,
+                          double* latency_ms);
+/* Wait until every submitted request has completed. */
+int32_t cwbvh_server_drain(cwbvh_server* s);
+
+typedef struct cwbvh_server_stats {
+    int64_t completed;
+    int64_t rays;
+    int64_t batches;
+    int32_t max_inflight_seen;    /* most batches on the device at once */
+    int32_t max_batch_requests_seen;
+    int64_t peak_bytes;           /* peak device bytes held by in-flight requests */
+    double busy_s;                /* wall time with at least one batch in flight */
+} cwbvh_server_stats;
+
+int32_t cwbvh_server_get_stats(const cwbvh_server* s, cwbvh_server_stats* out);
+void cwbvh_server_destroy(cwbvh_server* s);
+
+/* Poisson load generator: num_requests requests at `rate` per second, rays per
+ * request log-uniform in [min_rays, max_rays], scene uniform over the
+ * server's scenes.  Request k's rays are a window of ray_pools[scene]
+ * (pinned host arrays of pool_sizes[scene] rays); hits land in hit_pool
+ * (pinned, hit_pool_size hits, used as a ring).  latencies_ms (nullable,
+ * num_requests doubles) receives each request's latency in arrival order. */
+typedef struct cwbvh_poisson_spec {
+    double rate;
+    int32_t num_requests;
+    int64_t min_rays;
+    int64_t max_rays;
+    uint64_t seed;
+} cwbvh_poisson_spec;
+
+typedef struct cwbvh_poisson_result {
+    double wall_s;
+    double p50_ms;
+    double p99_ms;
+    double mean_ms;
+    double max_ms;
+    int64_t rays;
+    double mrays_per_s;
+} cwbvh_poisson_result;
+
+int32_t cwbvh_server_run_poisson(cwbvh_server* s, const cwbvh_poisson_spec* spec,
+                                 const cwbvh_ray* const* ray_pools, const int64_t* pool_sizes,
+                                 cwbvh_hit* hit_pool, int64_t hit_pool_size,
+                                 double* latencies_ms, cwbvh_poisson_result* out);
+
+/* ---------------------------------------------------------- autotuner --- */
+
+typedef struct cwbvh_tune_space {
+    int32_t block_sizes[4];
+    int32_t n_block;
+    int32_t lds_depths[4];
+    int32_t n_lds;
+    int32_t morton[2];         /* 0 = rays as given, 1 = Morton pre-pass */
+    int32_t n_morton;
+    int32_t prefetch[2];       /* node-prefetch distance 0 or 1 */
+    int32_t n_prefetch;
+} cwbvh_tune_space;
+
+typedef struct cwbvh_tune_record {
+    cwbvh_trace_config cfg;
+    int32_t morton;
+    float ms;                  /* median over reps: pre-pass + traversal */
+} cwbvh_tune_record;
+
+/* Time every point of `space` on the batch (median of `reps` launches after
+ * one warm-up) and return the fastest.  records (nullable) receives up to
+ * capacity entries; *n_records the number of points measured. */
+int32_t cwbvh_autotune_batch(const cwbvh_batch_item* items, int32_t count, uint32_t flags,
+                             const cwbvh_tune_space* space, int32_t reps,
+                             cwbvh_tune_record* records, int32_t capacity, int32_t* n_records,
+                             cwbvh_tune_record* best, void* stream);
 
 /* ------------------------------------ reference entry points (drop-in) ---
  * Same names and argument meaning as the reference's
