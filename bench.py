@@ -180,6 +180,88 @@ def bench_c2(args, cw, W, dev) -> int:
     return 0
 
 
+def bench_c3(args, cw, W, dev) -> int:
+    """C3: large incoherent scene, closest hit; rays shuffled vs Morton-ordered
+    (Morton order applied through the ordered-trace indirection; the sort is
+    timed separately)."""
+    t0 = time.perf_counter()
+    mesh = W.c3_mesh(args.c3_tris)
+    diag = mesh.diagonal()
+    rays = cw.rays_sphere_from_surface(mesh, args.c3_rays, 1e-4 * diag, 0.0, np.inf, seed=W.SEEDS["C3"] + 1)
+    rays = cw.shuffle_rays(rays, seed=W.SEEDS["C3"] + 2)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bvh2, cwb = cw.build_layouts(mesh)
+    build_s = time.perf_counter() - t0
+    n = len(rays)
+    log(f"C3: {mesh.num_triangles} tris, {n} rays, gen {gen_s:.1f} s, build {build_s:.1f} s, "
+        f"cwbvh {cwb.stats()}, bvh2 {bvh2.stats()}")
+    s_cw, s_b2 = cw.Scene(cwb), cw.Scene(bvh2)
+    lo, hi = s_cw.bounds()
+    del bvh2
+    dr = cw.DeviceBuffer.from_numpy(rays)
+    del rays
+    dh = cw.DeviceBuffer(16 * n)
+    do = cw.DeviceBuffer(4 * n)
+    e0, e1 = cw.Event(), cw.Event()
+    e0.record()
+    cw.morton_order(dr.ptr, n, lo, hi, do.ptr)
+    e1.record()
+    e1.synchronize()
+    sort_ms = e0.elapsed_ms(e1)
+    cfg_cw = cw.default_trace_config(cw.KIND_CWBVH)
+    cfg_b2 = cw.default_trace_config(cw.KIND_BVH2)
+
+    def t(sc, cfg, order):
+        fn = lambda: sc.trace(dr.ptr, dh.ptr, n, 0, cfg, None, None, order)
+        total, per = time_launches([fn], args.steps, args.warmup, cw)
+        return float(per[:, 0].mean())
+
+    res = {}
+    for name, sc, cfg in (("cwbvh", s_cw, cfg_cw), ("bvh2", s_b2, cfg_b2)):
+        for oname, order in (("shuffled", None), ("morton", do.ptr)):
+            ms = t(sc, cfg, order)
+            res[f"{name}_{oname}"] = {"ms": round(ms, 3), "mrays_per_s": round(n / ms / 1e3, 1)}
+            log(f"{name} {oname}: {ms:.2f} ms {n / ms / 1e3:.0f} Mrays/s")
+    d_cnt = cw.DeviceBuffer(16)
+    d_cnt.zero()
+    s_cw.trace(dr.ptr, dh.ptr, n, cw.TRACE_STATS, cfg_cw, d_cnt.ptr)
+    cw.synchronize()
+    c = d_cnt.download((2,), np.uint64)
+    bytes_c = algorithmic_bytes(1, n, int(c[0]), int(c[1]))
+    ms_c = res["cwbvh_shuffled"]["ms"]
+    achieved = bytes_c / (ms_c / 1e3) / 1e9
+    value = n / ms_c / 1e3
+    out = {
+        "metric": METRIC + " [C3 large incoherent]",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_c,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"C3: clustered soup {mesh.num_triangles} triangles (4096 clusters), {n} diffuse "
+                               f"rays from surface points, tmax=inf, closest hit, shuffled order",
+                   "triangles": mesh.num_triangles, "rays": n, "seed": W.SEEDS["C3"]},
+        "results": res,
+        "morton_sort_ms": round(sort_ms, 3),
+        "cwbvh_vs_bvh2": {"shuffled": round(res["bvh2_shuffled"]["ms"] / res["cwbvh_shuffled"]["ms"], 3),
+                          "morton": round(res["bvh2_morton"]["ms"] / res["cwbvh_morton"]["ms"], 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "per_ray": {"nodes": int(c[0]) / n, "triangles": int(c[1]) / n, "bytes": bytes_c / n}},
+        "build_s": round(build_s, 1),
+        "device": dev["arch"],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_c4(args, cw, W, dev) -> int:
     """C4: resident scenes, Poisson AO query requests (log-uniform 1k..1M rays,
     uniform scene), host-resident rays copied in and hits copied out per
@@ -265,7 +347,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c4"])
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3", "c4"])
+    ap.add_argument("--c3-tris", type=int, default=20_000_000)
+    ap.add_argument("--c3-rays", type=int, default=64 * 1024 * 1024)
     ap.add_argument("--c4-scenes", type=int, default=32)
     ap.add_argument("--c4-tris", type=int, default=100_000_000, help="total triangles of the C4 scene set")
     ap.add_argument("--c4-requests", type=int, default=400)
@@ -296,6 +380,8 @@ def main() -> int:
         return bench_c2(args, cw, W, dev)
     if args.config == "c4":
         return bench_c4(args, cw, W, dev)
+    if args.config == "c3":
+        return bench_c3(args, cw, W, dev)
 
     params = cw.default_build_params(cwbvh_node_stride=args.node_stride)
     t0 = time.perf_counter()

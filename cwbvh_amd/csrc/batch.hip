@@ -203,6 +203,9 @@ int32_t cwbvh_batch_create(const cwbvh_batch_item* items, int32_t count, uint32_
     else cwbvh_trace_config_default(CWBVH_KIND_CWBVH, &c);
     int32_t rc = validate_config(CWBVH_KIND_CWBVH, c);
     if (rc) return rc;
+    // the job table is planned in chunks: fix the chunk size here so that the
+    // plan and the kernel use the same value
+    if (c.fetch_chunk == 0) c.fetch_chunk = kBatchChunk;
     auto* b = new cwbvh_batch();
     b->flags = flags;
     b->cfg = c;

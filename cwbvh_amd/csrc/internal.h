@@ -92,6 +92,10 @@ struct BatchLaunch {
     unsigned* counters;
     hipStream_t stream;
 };
+// Rays per job chunk of a multi-scene launch when the configuration asks for
+// an automatic chunk size (fetch_chunk == 0); batch plans and the kernel must
+// agree on it.
+constexpr int32_t kBatchChunk = 256;
 int32_t launch_cwbvh_batch(const BatchLaunch& L);
 
 int32_t validate_config(int32_t kind, const cwbvh_trace_config& c);

@@ -338,6 +338,7 @@ int32_t cwbvh_server_create(const cwbvh_scene* const* scenes, int32_t num_scenes
         CW_CHECK_ARG(scenes[i] && scenes[i]->kind == CWBVH_KIND_CWBVH &&
                          (!(c.flags & CWBVH_TRACE_PRIM_ID) || scenes[i]->prim_idx),
                      "server scenes must be CWBVH scenes");
+    if (c.trace.fetch_chunk == 0) c.trace.fetch_chunk = kBatchChunk;  // batches are planned in chunks
     auto* s = new cwbvh_server();
     s->scenes.assign(scenes, scenes + num_scenes);
     s->cfg = c;

@@ -87,8 +87,10 @@ __device__ __forceinline__ unsigned child_hits4(unsigned meta4, unsigned lox, un
         const float tfx = __builtin_fmaf((float)((hix >> sh) & 0xFFu), adjx, origx);
         const float tfy = __builtin_fmaf((float)((hiy >> sh) & 0xFFu), adjy, origy);
         const float tfz = __builtin_fmaf((float)((hiz >> sh) & 0xFFu), adjz, origz);
-        const float cmin = fmaxf(imax3(tnx, tny, tnz), tmin);
-        const float cmax = fminf(imin3(tfx, tfy, tfz), tmax);
+        // int-ordered combine with tmin/tmax as well: equal to fmaxf/fminf for
+        // tmin, tmax >= +0 and non-NaN slabs, without NaN canonicalization ops
+        const float cmin = imax2(imax3(tnx, tny, tnz), tmin);
+        const float cmax = imin2(imin3(tfx, tfy, tfz), tmax);
         const unsigned bits = (child_bits4 >> sh) & 0xFFu;
         const unsigned idx = (bit_index4 >> sh) & 0xFFu;
         hitmask |= (cmin <= cmax) ? (bits << idx) : 0u;
@@ -113,7 +115,7 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
     int stride = P.stride;
     float3 o = make_float3(0.f, 0.f, 0.f), d = o;
     float tmin = 0.f, tmax = 0.f, idx = 0.f, idy = 0.f, idz = 0.f;
-    unsigned octinv = 0;
+    unsigned octinv = 0, octinv4 = 0;
     uint2 ng = make_uint2(0u, 0u), tg = make_uint2(0u, 0u);
     int sp = 0;
     int hit_addr = -1;
@@ -241,6 +243,7 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
             idz = safe_rcp(d.z);
             const unsigned oct = ((d.x < 0.f ? 1u : 0u) << 2) | ((d.y < 0.f ? 1u : 0u) << 1) | (d.z < 0.f ? 1u : 0u);
             octinv = 7u - oct;
+            octinv4 = octinv * 0x01010101u;
             ng = make_uint2(0u, 0x80000000u);
             tg = make_uint2(0u, 0u);
             sp = 0;
@@ -286,7 +289,6 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
                 const float origx = -(o.x - n0.x) * idx;
                 const float origy = -(o.y - n0.y) * idy;
                 const float origz = -(o.z - n0.z) * idz;
-                const unsigned octinv4 = octinv * 0x01010101u;
                 const bool sgx = idx < 0.f, sgy = idy < 0.f, sgz = idz < 0.f;
                 unsigned hitmask;
                 {
@@ -557,9 +559,7 @@ int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_
     p.total_waves = (int)(grid * (block / 64));
     if (p.chunk <= 0) {
         // automatic: about 16 claims per wave over the launch, 64..1024 rays each
-        const long long rays = MULTI ? 0 : p.n;
-        long long c = rays / ((long long)p.total_waves * 16);
-        if (MULTI) c = 256;
+        long long c = (long long)p.n / ((long long)p.total_waves * 16);
         c = std::min<long long>(1024, std::max<long long>(64, (c + 63) / 64 * 64));
         p.chunk = (int)c;
     }
@@ -652,6 +652,10 @@ int32_t launch_cwbvh(const TraceLaunch& L) {
 }
 
 int32_t launch_cwbvh_batch(const BatchLaunch& L) {
+    if (L.cfg.fetch_chunk <= 0) {
+        set_error("batch launch needs the chunk size its job table was planned with");
+        return CWBVH_ERR_INVALID_ARGUMENT;
+    }
     CwbvhParams p{};
     p.jobs = L.d_jobs;
     p.num_jobs = L.num_jobs;

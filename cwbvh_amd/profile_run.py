@@ -1,0 +1,56 @@
+"""Minimal launch sequence for hardware-counter runs under rocprofv3.
+
+    rocprofv3 --pmc FETCH_SIZE -- python3 -m cwbvh_amd.profile_run --workload c1
+
+Runs, in order: one HBM copy of --copy-mib MiB through `copy_u4_kernel`
+(calibrates the byte counters against a known size), then --launches
+CWBVH closest-hit launches and --launches BVH2 closest-hit launches over the
+workload's rays.  Prints one JSON line with the launch sizes so a counter
+summary can be matched to the kernels.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+
+import numpy as np
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--launches", type=int, default=2)
+    ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--any", action="store_true")
+    args = ap.parse_args()
+
+    import cwbvh_amd as cw
+    from cwbvh_amd import workloads as W
+
+    wl = W.c0() if args.workload == "c0" else W.c1(W.gpu_primary_tracer())
+    bvh2, cwb = cw.build_layouts(wl.mesh)
+    n = len(wl.rays)
+    s_cw, s_b2 = cw.Scene(cwb), cw.Scene(bvh2)
+    dr = cw.DeviceBuffer.from_numpy(wl.rays)
+    dh = cw.DeviceBuffer(16 * n)
+    copy_bytes = args.copy_mib << 20
+    src = cw.DeviceBuffer(copy_bytes)
+    dst = cw.DeviceBuffer(copy_bytes)
+    src.zero()
+    cw.synchronize()
+    cw.check(cw.lib.cwbvh_copy_u4(dst.ptr, src.ptr, copy_bytes // 16, None))
+    flags = cw.TRACE_ANY if args.any else cw.TRACE_CLOSEST
+    for _ in range(args.launches):
+        s_cw.trace(dr.ptr, dh.ptr, n, flags)
+    for _ in range(args.launches):
+        s_b2.trace(dr.ptr, dh.ptr, n, flags)
+    cw.synchronize()
+    print(json.dumps({"workload": wl.name, "rays": n, "copy_bytes": copy_bytes, "launches": args.launches,
+                      "flags": flags, "cwbvh_nodes": cwb.stats()["num_nodes"],
+                      "scene_bytes": {"cwbvh": s_cw.device_bytes(), "bvh2": s_b2.device_bytes()}}))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -161,8 +161,8 @@ static int trace_cwbvh_one(const or_float4* nodes, int stride, const or_float4* 
                     float tfx = fmaf((float)((fx >> sh) & 0xFF), adjx, origx);
                     float tfy = fmaf((float)((fy >> sh) & 0xFF), adjy, origy);
                     float tfz = fmaf((float)((fz >> sh) & 0xFF), adjz, origz);
-                    float cmin = fmaxf(imax3(tnx, tny, tnz), r.tmin);
-                    float cmax = fminf(imin3(tfx, tfy, tfz), tmax);
+                    float cmin = imax2(imax3(tnx, tny, tnz), r.tmin);
+                    float cmax = imin2(imin3(tfx, tfy, tfz), tmax);
                     if (cmin <= cmax) {
                         uint32_t meta = (meta4 >> sh) & 0xFF;
                         uint32_t is_inner = (meta & (meta << 1)) & 0x10u;

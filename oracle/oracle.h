@@ -9,7 +9,9 @@
  *   idir_k  = 1 / (|d_k| > 2^-80 ? d_k : copysign(2^-80, d_k))
  *   CWBVH   adj_k  = 2^e_k * idir_k ; orig_k = (-(o_k - p_k)) * idir_k
  *           slab   = fma(q, adj_k, orig_k)
- *           cmin   = fmaxf(max3_int(near slabs), tmin); cmax = fminf(min3_int(far slabs), tmax)
+ *           cmin   = max_int(max3_int(near slabs), tmin); cmax = min_int(min3_int(far slabs), tmax)
+ *           (the reference combines with tmin/tmax through fmaxf/fminf; for
+ *            tmin, tmax >= +0 and non-NaN slabs both orders give the same value)
  *   BVH2    ood_k  = o_k * idir_k ; slab = fma(bound, idir_k, -ood_k)
  *           tmin'  = max_int(min_int(lo,hi) per axis, tmin); tmax' = min_int(max_int(lo,hi) per axis, tmax)
  *   (max/min_int = order of the IEEE bit patterns read as int32, the reference's
