@@ -386,6 +386,8 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
         atomicAdd(&P.stats[1], st_tris);
     }
     // last wave out resets the fetch counters for the next launch on this stream
+    // (one-ray-per-thread launches claim nothing, so they skip the exit count)
+    if (!MULTI && !P.persistent) return;
     __threadfence();
     const unsigned long long alive = __ballot(1);
     if (lane == __ffsll((long long)alive) - 1) {
@@ -558,9 +560,9 @@ int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_
     }
     p.total_waves = (int)(grid * (block / 64));
     if (p.chunk <= 0) {
-        // automatic: about 16 claims per wave over the launch, 64..1024 rays each
-        long long c = (long long)p.n / ((long long)p.total_waves * 16);
-        c = std::min<long long>(1024, std::max<long long>(64, (c + 63) / 64 * 64));
+        // automatic: about 32 claims per wave over the launch, 64..512 rays each
+        long long c = (long long)p.n / ((long long)p.total_waves * 32);
+        c = std::min<long long>(512, std::max<long long>(64, (c + 63) / 64 * 64));
         p.chunk = (int)c;
     }
     hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(block), lds, stream, p);

@@ -79,7 +79,7 @@ def test_budget_admission(cw, serving_set):
     srv = cw.Server(scenes, cw.default_server_config(memory_budget=budget, max_inflight=4))
     big = cw.PinnedArray((200000, 4), np.float32)
     with pytest.raises(cw.CwbvhError) as e:
-        srv.submit(0, pools[0].ptr, 100000, big.ptr)  # 4.8 MB > budget
+        srv.submit(0, pools[0].ptr, 200000, big.ptr)  # 9.6 MB > 6 MB budget
     assert e.value.status == 7
     outs = []
     for k in range(12):
