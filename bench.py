@@ -140,7 +140,7 @@ def bench_c2(args, cw, W, dev) -> int:
     log(f"plain {ms_plain:.2f} ms, plain+sync {ms_plain_sync:.2f} ms")
     t = time.perf_counter()
     best, recs = cw.autotune_batch(items, block_sizes=(64, 128, 256), lds_depths=(8, 16, 32), morton=(0, 1),
-                                   prefetch=(0, 1), reps=args.tune_reps, stream=st)
+                                   prefetch=(0, 1), loop_modes=(0, 1, 2), reps=args.tune_reps, stream=st)
     tune_s = time.perf_counter() - t
     log(f"autotune {tune_s:.1f} s best {best.as_dict()}")
     batch = cw.Batch(items, cw.TRACE_CLOSEST, best.cfg, morton=bool(best.morton))

@@ -85,6 +85,7 @@ class TraceConfig(C.Structure):
         ("prefetch", C.c_int32),
         ("work_slices", C.c_int32),
         ("persistent", C.c_int32),
+        ("loop_mode", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -101,6 +102,7 @@ class TuneSpace(C.Structure):
         ("lds_depths", C.c_int32 * 4), ("n_lds", C.c_int32),
         ("morton", C.c_int32 * 2), ("n_morton", C.c_int32),
         ("prefetch", C.c_int32 * 2), ("n_prefetch", C.c_int32),
+        ("loop_modes", C.c_int32 * 3), ("n_loop_modes", C.c_int32),
     ]
 
 
@@ -419,12 +421,20 @@ def shuffle_rays(rays: np.ndarray, seed: int = 0) -> np.ndarray:
 # ------------------------------------------------------------------ layouts
 
 
+def _set_fields(struct, overrides: dict):
+    """Apply keyword overrides to a ctypes struct, rejecting unknown names."""
+    names = {f for f, _ in type(struct)._fields_}
+    for k, v in overrides.items():
+        if k not in names:
+            raise TypeError(f"{type(struct).__name__} has no field {k!r}")
+        setattr(struct, k, v)
+    return struct
+
+
 def default_build_params(**overrides) -> BuildParams:
     p = BuildParams()
     check(lib.cwbvh_build_params_default(C.byref(p)))
-    for k, v in overrides.items():
-        setattr(p, k, v)
-    return p
+    return _set_fields(p, overrides)
 
 
 class Layout:
@@ -627,9 +637,7 @@ def synchronize() -> None:
 def default_trace_config(kind: int, **overrides) -> TraceConfig:
     c = TraceConfig()
     check(lib.cwbvh_trace_config_default(kind, C.byref(c)))
-    for k, v in overrides.items():
-        setattr(c, k, v)
-    return c
+    return _set_fields(c, overrides)
 
 
 class Scene:
@@ -738,17 +746,19 @@ class Batch:
 
 
 def autotune_batch(items, flags: int = TRACE_CLOSEST, block_sizes=(64, 128, 256), lds_depths=(8, 16, 32),
-                   morton=(0, 1), prefetch=(0,), reps: int = 3, stream: Stream | None = None):
-    """Time every configuration of the space on the batch; returns (best, records)."""
+                   morton=(0, 1), prefetch=(0,), loop_modes=(), reps: int = 3, stream: Stream | None = None):
+    """Time every configuration of the space on the batch; returns (best, records).
+    An empty loop_modes keeps the default inner-loop schedule."""
     sp = TuneSpace()
-    for name, vals, cap in (("block_sizes", block_sizes, 4), ("lds_depths", lds_depths, 4),
-                            ("morton", morton, 2), ("prefetch", prefetch, 2)):
-        assert 1 <= len(vals) <= cap
+    for name, vals, lo, cap in (("block_sizes", block_sizes, 1, 4), ("lds_depths", lds_depths, 1, 4),
+                                ("morton", morton, 1, 2), ("prefetch", prefetch, 1, 2), ("loop_modes", loop_modes, 0, 3)):
+        assert lo <= len(vals) <= cap
         arr = getattr(sp, name)
         for i, v in enumerate(vals):
             arr[i] = v
     sp.n_block, sp.n_lds, sp.n_morton, sp.n_prefetch = len(block_sizes), len(lds_depths), len(morton), len(prefetch)
-    cap = 4 * 4 * 2 * 2
+    sp.n_loop_modes = len(loop_modes)
+    cap = 4 * 4 * 2 * 2 * 3
     recs = (TuneRecord * cap)()
     nrec = C.c_int32()
     best = TuneRecord()
@@ -785,9 +795,7 @@ class PinnedArray:
 def default_server_config(**overrides) -> ServerConfig:
     c = ServerConfig()
     check(lib.cwbvh_server_config_default(C.byref(c)))
-    for k, v in overrides.items():
-        setattr(c, k, v)
-    return c
+    return _set_fields(c, overrides)
 
 
 class Server:

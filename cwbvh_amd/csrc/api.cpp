@@ -361,17 +361,20 @@ int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* c) {
     if (kind == CWBVH_KIND_CWBVH) {
         // measured on MI355X (C1, 33M AO rays): 10 LDS entries cover the
         // stack of trees up to depth 9 at full occupancy; ~256-ray claims
-        // keep the fetch atomics off the critical path
+        // keep the fetch atomics off the critical path; the interleaved
+        // schedule is 1.36x the grouped one and keeps the reference's
+        // per-ray order (speculation adds under 3%)
         c->block_size = 128;
         c->lds_depth = 10;
         c->grid_blocks = 0;
         c->fetch_chunk = 0;   // automatic
-        c->refill_idle = 8;
-        c->refill_budget = 32;
+        c->refill_idle = 4;
+        c->refill_budget = 16;
         c->postpone_pct = 0;
         c->prefetch = 0;
         c->work_slices = 1;
         c->persistent = 1;
+        c->loop_mode = 1;
     } else {
         c->block_size = 256;
         c->lds_depth = 8;

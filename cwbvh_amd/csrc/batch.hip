@@ -368,7 +368,8 @@ int32_t cwbvh_autotune_batch(const cwbvh_batch_item* items, int32_t count, uint3
                              int32_t capacity, int32_t* n_records, cwbvh_tune_record* best, void* stream) {
     CW_CHECK_ARG(space != nullptr && best != nullptr, "NULL argument");
     CW_CHECK_ARG(space->n_block >= 1 && space->n_block <= 4 && space->n_lds >= 1 && space->n_lds <= 4 &&
-                     space->n_morton >= 1 && space->n_morton <= 2 && space->n_prefetch >= 1 && space->n_prefetch <= 2,
+                     space->n_morton >= 1 && space->n_morton <= 2 && space->n_prefetch >= 1 && space->n_prefetch <= 2 &&
+                     space->n_loop_modes >= 0 && space->n_loop_modes <= 3,
                  "bad tuning space");
     CW_CHECK_ARG(reps >= 1 && reps <= 1000, "reps must be in [1, 1000]");
     hipStream_t st = (hipStream_t)stream;
@@ -381,12 +382,14 @@ int32_t cwbvh_autotune_batch(const cwbvh_batch_item* items, int32_t count, uint3
     for (int ib = 0; ib < space->n_block && rc == CWBVH_OK; ++ib)
         for (int il = 0; il < space->n_lds && rc == CWBVH_OK; ++il)
             for (int im = 0; im < space->n_morton && rc == CWBVH_OK; ++im)
-                for (int ip = 0; ip < space->n_prefetch && rc == CWBVH_OK; ++ip) {
+                for (int ip = 0; ip < space->n_prefetch && rc == CWBVH_OK; ++ip)
+                for (int io = 0; io < std::max(1, space->n_loop_modes) && rc == CWBVH_OK; ++io) {
                     cwbvh_trace_config c;
                     cwbvh_trace_config_default(CWBVH_KIND_CWBVH, &c);
                     c.block_size = space->block_sizes[ib];
                     c.lds_depth = space->lds_depths[il];
                     c.prefetch = space->prefetch[ip];
+                    if (space->n_loop_modes > 0) c.loop_mode = space->loop_modes[io];
                     cwbvh_batch* b = nullptr;
                     rc = cwbvh_batch_create(items, count, flags, &c, space->morton[im], &b);
                     if (rc) break;

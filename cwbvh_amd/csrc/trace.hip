@@ -98,7 +98,23 @@ __device__ __forceinline__ unsigned child_hits4(unsigned meta4, unsigned lox, un
     return hitmask;
 }
 
-template <bool ANY, bool STATS, bool MULTI, bool PREFETCH>
+// Inner-loop schedules (cwbvh_trace_config.loop_mode).
+//  kLoopGroup: per iteration one node visit, then every triangle of the
+//    group it produced (the reference's order, :93-340), with optional
+//    postponing of the triangle group.
+//  kLoopInterleave: per iteration a lane either visits a node (when it holds
+//    no triangles) or tests ONE triangle; the per-ray sequence of visits and
+//    tests is the same as kLoopGroup, but a lane with a long triangle group no
+//    longer holds the other 63 lanes in a triangle loop.
+//  kLoopSpeculate: as kLoopInterleave, and a lane that holds triangles still
+//    visits a node in the same iteration; the new node's triangle group is
+//    pushed on the stack and tested later, so node visits keep every lane
+//    with inner children busy.
+constexpr int kLoopGroup = 0;
+constexpr int kLoopInterleave = 1;
+constexpr int kLoopSpeculate = 2;
+
+template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP>
 __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
     extern __shared__ unsigned long long lds_stack_raw[];
     CWD_LDS unsigned long long* lds_stack = lds_ptr(lds_stack_raw);
@@ -253,117 +269,163 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
             busy = true;
         }
 
+        // stack push of a node or triangle group (overflow is flagged, not written)
+        auto push = [&](uint2 e) {
+            if (sp < kStackCapacity) {
+                stack_put(lds_stack, spill, sx, sp, pack2(e));
+                ++sp;
+            } else {
+                P.work[kWorkOverflow] = 1u;
+            }
+        };
+        // Visit the highest-priority child of ng: the remainder of ng goes on
+        // the stack, ng becomes the child's inner children, the child's leaf
+        // triangles are returned.
+        auto visit_node = [&]() -> uint2 {
+            const unsigned imask = ng.y;
+            const int bit = 31 - __clz((int)ng.y);
+            const unsigned base = ng.x;
+            ng.y &= ~(1u << bit);
+            if (ng.y > 0x00FFFFFFu) push(ng);
+            float4 n0, n1, n2, n3, n4;
+            if (PREFETCH && have_pf) {
+                n0 = pf0; n1 = pf1; n2 = pf2; n3 = pf3; n4 = pf4;
+                have_pf = false;
+            } else {
+                const unsigned slot = (unsigned)(bit - 24) ^ octinv;
+                const unsigned rel = (unsigned)__popc(imask & ~(0xFFFFFFFFu << slot));
+                const float4* np = nodes + (size_t)(base + rel) * (unsigned)stride;
+                n0 = np[0]; n1 = np[1]; n2 = np[2]; n3 = np[3]; n4 = np[4];
+            }
+            if (STATS) ++st_nodes;
+            const unsigned eb = f2u(n0.w);
+            const int ex = (int)(signed char)(eb & 0xFFu);
+            const int ey = (int)(signed char)((eb >> 8) & 0xFFu);
+            const int ez = (int)(signed char)((eb >> 16) & 0xFFu);
+            const float adjx = u2f((unsigned)(ex + 127) << 23) * idx;
+            const float adjy = u2f((unsigned)(ey + 127) << 23) * idy;
+            const float adjz = u2f((unsigned)(ez + 127) << 23) * idz;
+            const float origx = -(o.x - n0.x) * idx;
+            const float origy = -(o.y - n0.y) * idy;
+            const float origz = -(o.z - n0.z) * idz;
+            const bool sgx = idx < 0.f, sgy = idy < 0.f, sgz = idz < 0.f;
+            unsigned hitmask;
+            {
+                const unsigned qlx = f2u(n2.x), qly = f2u(n2.z), qlz = f2u(n3.x);
+                const unsigned qhx = f2u(n3.z), qhy = f2u(n4.x), qhz = f2u(n4.z);
+                hitmask = child_hits4(f2u(n1.z), sgx ? qhx : qlx, sgy ? qhy : qly, sgz ? qhz : qlz,
+                                      sgx ? qlx : qhx, sgy ? qly : qhy, sgz ? qlz : qhz, adjx, adjy, adjz,
+                                      origx, origy, origz, tmin, tmax, octinv4);
+            }
+            {
+                const unsigned qlx = f2u(n2.y), qly = f2u(n2.w), qlz = f2u(n3.y);
+                const unsigned qhx = f2u(n3.w), qhy = f2u(n4.y), qhz = f2u(n4.w);
+                hitmask |= child_hits4(f2u(n1.w), sgx ? qhx : qlx, sgy ? qhy : qly, sgz ? qhz : qlz,
+                                       sgx ? qlx : qhx, sgy ? qly : qhy, sgz ? qlz : qhz, adjx, adjy, adjz,
+                                       origx, origy, origz, tmin, tmax, octinv4);
+            }
+            ng.x = f2u(n1.x);
+            ng.y = (hitmask & 0xFF000000u) | (eb >> 24);
+            if (PREFETCH && ng.y > 0x00FFFFFFu) {
+                // the next node visit takes the highest hit child: start its
+                // loads now so they overlap the triangle tests in between
+                const int nb = 31 - __clz((int)ng.y);
+                const unsigned nslot = (unsigned)(nb - 24) ^ octinv;
+                const unsigned nrel = (unsigned)__popc(ng.y & ~(0xFFFFFFFFu << nslot));
+                const float4* np = nodes + (size_t)(ng.x + nrel) * (unsigned)stride;
+                pf0 = np[0]; pf1 = np[1]; pf2 = np[2]; pf3 = np[3]; pf4 = np[4];
+                have_pf = true;
+            }
+            return make_uint2(f2u(n1.y), hitmask & 0x00FFFFFFu);
+        };
+        // Test the highest triangle of tg; true when an any-hit query is done.
+        auto test_triangle = [&]() -> bool {
+            const int ti = 31 - __clz((int)tg.y);
+            const int addr = (int)(tg.x * 3u + (unsigned)ti * 3u);
+            const float4 v0 = tris[addr], v1 = tris[addr + 1], v2 = tris[addr + 2];
+            if (STATS) ++st_tris;
+            tg.y &= ~(1u << ti);
+            float t, u, v;
+            if (woop_test<ANY>(v0, v1, v2, o, d, tmin, tmax, addr, hit_addr, t, u, v)) {
+                tmax = t;
+                hit_addr = addr;
+                hu = u;
+                hv = v;
+                return ANY;
+            }
+            return false;
+        };
+
         int lost = 0;
         while (true) {
-            if (ng.y > 0x00FFFFFFu) {
-                const unsigned imask = ng.y;
-                const int bit = 31 - __clz((int)ng.y);
-                const unsigned base = ng.x;
-                ng.y &= ~(1u << bit);
-                if (ng.y > 0x00FFFFFFu) {
-                    if (sp < kStackCapacity) {
-                        stack_put(lds_stack, spill, sx, sp, pack2(ng));
-                        ++sp;
-                    } else {
-                        P.work[kWorkOverflow] = 1u;
-                    }
-                }
-                float4 n0, n1, n2, n3, n4;
-                if (PREFETCH && have_pf) {
-                    n0 = pf0; n1 = pf1; n2 = pf2; n3 = pf3; n4 = pf4;
-                    have_pf = false;
-                } else {
-                    const unsigned slot = (unsigned)(bit - 24) ^ octinv;
-                    const unsigned rel = (unsigned)__popc(imask & ~(0xFFFFFFFFu << slot));
-                    const float4* np = nodes + (size_t)(base + rel) * (unsigned)stride;
-                    n0 = np[0]; n1 = np[1]; n2 = np[2]; n3 = np[3]; n4 = np[4];
-                }
-                if (STATS) ++st_nodes;
-                const unsigned eb = f2u(n0.w);
-                const int ex = (int)(signed char)(eb & 0xFFu);
-                const int ey = (int)(signed char)((eb >> 8) & 0xFFu);
-                const int ez = (int)(signed char)((eb >> 16) & 0xFFu);
-                const float adjx = u2f((unsigned)(ex + 127) << 23) * idx;
-                const float adjy = u2f((unsigned)(ey + 127) << 23) * idy;
-                const float adjz = u2f((unsigned)(ez + 127) << 23) * idz;
-                const float origx = -(o.x - n0.x) * idx;
-                const float origy = -(o.y - n0.y) * idy;
-                const float origz = -(o.z - n0.z) * idz;
-                const bool sgx = idx < 0.f, sgy = idy < 0.f, sgz = idz < 0.f;
-                unsigned hitmask;
-                {
-                    const unsigned qlx = f2u(n2.x), qly = f2u(n2.z), qlz = f2u(n3.x);
-                    const unsigned qhx = f2u(n3.z), qhy = f2u(n4.x), qhz = f2u(n4.z);
-                    hitmask = child_hits4(f2u(n1.z), sgx ? qhx : qlx, sgy ? qhy : qly, sgz ? qhz : qlz,
-                                          sgx ? qlx : qhx, sgy ? qly : qhy, sgz ? qlz : qhz, adjx, adjy, adjz,
-                                          origx, origy, origz, tmin, tmax, octinv4);
-                }
-                {
-                    const unsigned qlx = f2u(n2.y), qly = f2u(n2.w), qlz = f2u(n3.y);
-                    const unsigned qhx = f2u(n3.w), qhy = f2u(n4.y), qhz = f2u(n4.w);
-                    hitmask |= child_hits4(f2u(n1.w), sgx ? qhx : qlx, sgy ? qhy : qly, sgz ? qhz : qlz,
-                                           sgx ? qlx : qhx, sgy ? qly : qhy, sgz ? qlz : qhz, adjx, adjy, adjz,
-                                           origx, origy, origz, tmin, tmax, octinv4);
-                }
-                ng.x = f2u(n1.x);
-                ng.y = (hitmask & 0xFF000000u) | (eb >> 24);
-                tg.x = f2u(n1.y);
-                tg.y = hitmask & 0x00FFFFFFu;
-                if (PREFETCH && ng.y > 0x00FFFFFFu) {
-                    // the next iteration visits the highest hit child: start its
-                    // loads now so they overlap this node's triangle tests
-                    const int nb = 31 - __clz((int)ng.y);
-                    const unsigned nslot = (unsigned)(nb - 24) ^ octinv;
-                    const unsigned nrel = (unsigned)__popc(ng.y & ~(0xFFFFFFFFu << nslot));
-                    const float4* np = nodes + (size_t)(ng.x + nrel) * (unsigned)stride;
-                    pf0 = np[0]; pf1 = np[1]; pf2 = np[2]; pf3 = np[3]; pf4 = np[4];
-                    have_pf = true;
-                }
-            } else {
-                tg = ng;
-                ng = make_uint2(0u, 0u);
-            }
-
             bool finished = false;
-            const int total = P.postpone_pct > 0 ? __popcll(__ballot(1)) : 0;
-            while (tg.y != 0u) {
-                if (P.postpone_pct > 0) {
-                    const int active = __popcll(__ballot(1));
-                    if (active * 100 < total * P.postpone_pct) {
-                        if (sp < kStackCapacity) {
-                            stack_put(lds_stack, spill, sx, sp, pack2(tg));
-                            ++sp;
-                        } else {
-                            P.work[kWorkOverflow] = 1u;
-                        }
-                        tg.y = 0u;
-                        break;
-                    }
+            if constexpr (LOOP == kLoopGroup) {
+                if (ng.y > 0x00FFFFFFu) {
+                    tg = visit_node();
+                } else {
+                    // a postponed triangle group came off the stack
+                    tg = ng;
+                    ng = make_uint2(0u, 0u);
                 }
-                const int ti = 31 - __clz((int)tg.y);
-                const int addr = (int)(tg.x * 3u + (unsigned)ti * 3u);
-                const float4 v0 = tris[addr], v1 = tris[addr + 1], v2 = tris[addr + 2];
-                if (STATS) ++st_tris;
-                float t, u, v;
-                if (woop_test<ANY>(v0, v1, v2, o, d, tmin, tmax, addr, hit_addr, t, u, v)) {
-                    tmax = t;
-                    hit_addr = addr;
-                    hu = u;
-                    hv = v;
-                    if (ANY) {
+                const int total = P.postpone_pct > 0 ? __popcll(__ballot(1)) : 0;
+                while (tg.y != 0u) {
+                    if (P.postpone_pct > 0) {
+                        const int active = __popcll(__ballot(1));
+                        if (active * 100 < total * P.postpone_pct) {
+                            push(tg);
+                            tg.y = 0u;
+                            break;
+                        }
+                    }
+                    if (test_triangle()) {
                         finished = true;
                         break;
                     }
                 }
-                tg.y &= ~(1u << ti);
-            }
-
-            if (!finished && ng.y <= 0x00FFFFFFu) {
-                if (sp > 0) {
-                    --sp;
-                    ng = unpack2(stack_get(lds_stack, spill, sx, sp));
-                } else {
-                    finished = true;
+                if (!finished && ng.y <= 0x00FFFFFFu) {
+                    if (sp > 0) {
+                        --sp;
+                        ng = unpack2(stack_get(lds_stack, spill, sx, sp));
+                    } else {
+                        finished = true;
+                    }
+                }
+            } else {
+                // node step: lanes without triangles (and, speculating, every
+                // lane with inner children left) visit one node
+                const bool node_step = (LOOP == kLoopSpeculate || tg.y == 0u) && ng.y > 0x00FFFFFFu;
+                if (node_step) {
+                    const uint2 ntg = visit_node();
+                    if (LOOP == kLoopSpeculate && tg.y != 0u) {
+                        if (ntg.y != 0u) push(ntg);
+                    } else {
+                        tg = ntg;
+                    }
+                }
+                // triangle step: one test
+                if (tg.y != 0u) finished = test_triangle();
+                // out of inner children: take the next group off the stack
+                if (!finished && ng.y <= 0x00FFFFFFu) {
+                    if (tg.y == 0u) {
+                        if (sp > 0) {
+                            --sp;
+                            const uint2 e = unpack2(stack_get(lds_stack, spill, sx, sp));
+                            if (e.y > 0x00FFFFFFu) {
+                                ng = e;
+                            } else {
+                                tg = e;
+                                ng = make_uint2(0u, 0u);
+                            }
+                        } else {
+                            finished = true;
+                        }
+                    } else if (LOOP == kLoopSpeculate && sp > 0) {
+                        const uint2 e = unpack2(stack_get(lds_stack, spill, sx, sp - 1));
+                        if (e.y > 0x00FFFFFFu) {
+                            --sp;
+                            ng = e;
+                        }
+                    }
                 }
             }
             if (finished) {
@@ -546,11 +608,11 @@ int resident_blocks(K kernel, int block, size_t lds) {
     return nb;
 }
 
-template <bool ANY, bool STATS, bool MULTI, bool PREFETCH>
+template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP>
 int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t stream) {
     const int block = cfg.block_size;
     const size_t lds = (size_t)block * (size_t)cfg.lds_depth * sizeof(unsigned long long);
-    auto kernel = cwbvh_trace<ANY, STATS, MULTI, PREFETCH>;
+    auto kernel = cwbvh_trace<ANY, STATS, MULTI, PREFETCH, LOOP>;
     long long grid = cfg.grid_blocks;
     if (!MULTI && !p.persistent) {
         grid = ((long long)p.n + block - 1) / block;
@@ -580,20 +642,26 @@ int32_t launch_bvh2_t(Bvh2Params& p, const cwbvh_trace_config& cfg, hipStream_t 
     return CWBVH_OK;
 }
 
+template <bool ANY, bool STATS, bool MULTI, bool PREFETCH>
+int32_t dispatch_loop(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t st) {
+    switch (cfg.loop_mode) {
+        case kLoopInterleave: return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopInterleave>(p, cfg, st);
+        case kLoopSpeculate: return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopSpeculate>(p, cfg, st);
+        default: return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopGroup>(p, cfg, st);
+    }
+}
+
+template <bool ANY, bool STATS, bool MULTI>
+int32_t dispatch_prefetch(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t st) {
+    return cfg.prefetch ? dispatch_loop<ANY, STATS, MULTI, true>(p, cfg, st)
+                        : dispatch_loop<ANY, STATS, MULTI, false>(p, cfg, st);
+}
+
 template <bool MULTI>
 int32_t dispatch_cwbvh(CwbvhParams& p, uint32_t flags, const cwbvh_trace_config& cfg, hipStream_t st) {
     const bool any = (flags & CWBVH_TRACE_ANY) != 0, stats = (flags & CWBVH_TRACE_STATS) != 0;
-    const bool pf = cfg.prefetch != 0;
-    if (any) {
-        if (stats) return pf ? launch_cwbvh_t<true, true, MULTI, true>(p, cfg, st)
-                             : launch_cwbvh_t<true, true, MULTI, false>(p, cfg, st);
-        return pf ? launch_cwbvh_t<true, false, MULTI, true>(p, cfg, st)
-                  : launch_cwbvh_t<true, false, MULTI, false>(p, cfg, st);
-    }
-    if (stats) return pf ? launch_cwbvh_t<false, true, MULTI, true>(p, cfg, st)
-                         : launch_cwbvh_t<false, true, MULTI, false>(p, cfg, st);
-    return pf ? launch_cwbvh_t<false, false, MULTI, true>(p, cfg, st)
-              : launch_cwbvh_t<false, false, MULTI, false>(p, cfg, st);
+    if (any) return stats ? dispatch_prefetch<true, true, MULTI>(p, cfg, st) : dispatch_prefetch<true, false, MULTI>(p, cfg, st);
+    return stats ? dispatch_prefetch<false, true, MULTI>(p, cfg, st) : dispatch_prefetch<false, false, MULTI>(p, cfg, st);
 }
 
 }  // namespace
@@ -619,8 +687,13 @@ int32_t validate_config(int32_t kind, const cwbvh_trace_config& c) {
         if (c.fetch_chunk < 0 || c.fetch_chunk > (1 << 20) || c.grid_blocks < 0 || c.refill_budget < 1 ||
             c.persistent < 0 || c.persistent > 1 ||
             c.refill_idle < 0 || c.refill_idle > 64 || c.postpone_pct < 0 || c.postpone_pct > 100 ||
-            c.prefetch < 0 || c.prefetch > 1 || c.work_slices < 1 || c.work_slices > kWorkSlices) {
+            c.prefetch < 0 || c.prefetch > 1 || c.work_slices < 1 || c.work_slices > kWorkSlices ||
+            c.loop_mode < kLoopGroup || c.loop_mode > kLoopSpeculate) {
             set_error("invalid CWBVH trace configuration");
+            return CWBVH_ERR_INVALID_ARGUMENT;
+        }
+        if (c.postpone_pct > 0 && c.loop_mode != kLoopGroup) {
+            set_error("postpone_pct applies to loop_mode 0 (the other schedules test one triangle per iteration)");
             return CWBVH_ERR_INVALID_ARGUMENT;
         }
     }

@@ -265,6 +265,10 @@ typedef struct cwbvh_trace_config {
     int32_t prefetch;        /* 0 or 1: issue the next node's loads before the triangle loop */
     int32_t work_slices;     /* 1..8 contiguous ray ranges, one drained first per XCD (CWBVH) */
     int32_t persistent;      /* CWBVH: 1 = persistent grid with work fetching, 0 = one ray per thread */
+    int32_t loop_mode;       /* CWBVH inner-loop schedule: 0 = node visit + its whole triangle group
+                                per iteration (reference order), 1 = node visit or one triangle test
+                                per iteration, 2 = as 1, and lanes holding triangles also visit a
+                                node, parking its triangle group on the stack */
 } cwbvh_trace_config;
 
 int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
@@ -411,6 +415,8 @@ typedef struct cwbvh_tune_space {
     int32_t n_morton;
     int32_t prefetch[2];       /* node-prefetch distance 0 or 1 */
     int32_t n_prefetch;
+    int32_t loop_modes[3];     /* cwbvh_trace_config.loop_mode values */
+    int32_t n_loop_modes;      /* 0 = keep the default schedule */
 } cwbvh_tune_space;
 
 typedef struct cwbvh_tune_record {

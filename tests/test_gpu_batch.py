@@ -78,7 +78,9 @@ def test_batch_matches_oracle(cw, batch_set, morton, any_hit):
 
 @pytest.mark.parametrize("kw", [dict(block_size=64, lds_depth=4, fetch_chunk=1),
                                 dict(block_size=256, lds_depth=32, fetch_chunk=256, prefetch=1),
-                                dict(postpone_pct=0, refill_idle=0, refill_budget=1, grid_blocks=5)])
+                                dict(postpone_pct=0, refill_idle=0, refill_budget=1, grid_blocks=5),
+                                dict(loop_mode=1, block_size=64, lds_depth=2, fetch_chunk=3),
+                                dict(loop_mode=2, block_size=128, lds_depth=8, fetch_chunk=64, prefetch=1)])
 def test_batch_configs(cw, batch_set, kw):
     _, items, refs = batch_set
     bufs, ditems = device_items(cw, items)
@@ -153,8 +155,9 @@ def test_autotune_small(cw, batch_set):
     bufs, ditems = device_items(cw, items)
     say("autotune")
     best, recs = cw.autotune_batch(ditems, block_sizes=(64, 128), lds_depths=(8, 16), morton=(0, 1), prefetch=(0, 1),
-                                   reps=2)
-    assert len(recs) == 16 and best.ms == min(r.ms for r in recs) and best.ms > 0
+                                   loop_modes=(0, 2), reps=2)
+    assert len(recs) == 32 and best.ms == min(r.ms for r in recs) and best.ms > 0
+    assert {r.cfg.loop_mode for r in recs} == {0, 2}
     b = cw.Batch(ditems, cw.TRACE_CLOSEST, best.cfg, morton=bool(best.morton))
     b.launch()
     cw.synchronize()

@@ -81,11 +81,37 @@ CWBVH_VARIANTS = [
     dict(persistent=0, block_size=256, lds_depth=8, postpone_pct=0),
     dict(fetch_chunk=0, work_slices=8),
     dict(fetch_chunk=1024, work_slices=5),
+    dict(loop_mode=0),
+    dict(loop_mode=0, persistent=0, block_size=64, lds_depth=4),
+    dict(loop_mode=0, prefetch=1, refill_idle=8, refill_budget=32, fetch_chunk=33),
+    dict(loop_mode=1),
+    dict(loop_mode=1, persistent=0, block_size=64),
+    dict(loop_mode=1, prefetch=1, lds_depth=3, fetch_chunk=5),
+    dict(loop_mode=2),
+    dict(loop_mode=2, prefetch=1, lds_depth=1, fetch_chunk=1),
+    dict(loop_mode=2, block_size=256, refill_idle=0, refill_budget=1, work_slices=8),
+    dict(loop_mode=2, persistent=0, block_size=128, lds_depth=64),
 ]
 
 
+def assert_any_hit_valid(orc, lay, rays, g, ref, what):
+    """Any-hit returns the first hit its traversal order meets.  A schedule
+    that reorders triangle tests may report another triangle, but occlusion
+    (hit or miss) is order-independent and every reported hit is real."""
+    ids, rids = orc.hit_ids(g), orc.hit_ids(ref)
+    assert np.array_equal(ids >= 0, rids >= 0), f"{what}: occlusion differs on {int(((ids >= 0) != (rids >= 0)).sum())} rays"
+    same = ids == rids
+    assert_bitwise(g[same], ref[same], what)
+    for i in np.nonzero(~same)[0][:3000]:
+        ok, t, u, v = orc.woop_test(lay.woop[ids[i]:ids[i] + 3], rays[i])
+        assert ok and np.float32(t) == g[i, 0] and np.float32(u) == g[i, 2] and np.float32(v) == g[i, 3], (what, i)
+
+
 @pytest.mark.parametrize("variant", range(len(CWBVH_VARIANTS)))
-def test_cwbvh_config_variants_bit_identical(cw, c0, c0_ref, variant):
+def test_cwbvh_config_variants_bit_identical(cw, orc, c0, c0_ref, variant):
+    """Every schedule is bit-identical to the oracle for closest hit; for
+    any-hit too, except speculation (loop_mode 2), whose contract is
+    assert_any_hit_valid."""
     wl, lays = c0
     kw = CWBVH_VARIANTS[variant]
     say(f"CWBVH variant {kw}")
@@ -93,7 +119,11 @@ def test_cwbvh_config_variants_bit_identical(cw, c0, c0_ref, variant):
     sc = cw.Scene(lays["cwbvh"])
     for any_hit in (False, True):
         g = sc.trace_numpy(wl.rays, int(any_hit), cfg)
-        assert_bitwise(g, c0_ref[("cwbvh", any_hit)], f"variant {kw} any={any_hit}")
+        what = f"variant {kw} any={any_hit}"
+        if any_hit and kw.get("loop_mode", 0) == 2:
+            assert_any_hit_valid(orc, lays["cwbvh"], wl.rays, g, c0_ref[("cwbvh", True)], what)
+        else:
+            assert_bitwise(g, c0_ref[("cwbvh", any_hit)], what)
 
 
 @pytest.mark.parametrize("kw", [dict(block_size=64, lds_depth=1), dict(block_size=256, lds_depth=8),
@@ -108,7 +138,7 @@ def test_bvh2_config_variants_bit_identical(cw, c0, c0_ref, kw):
         assert_bitwise(g, c0_ref[("bvh2", any_hit)], f"bvh2 {kw} any={any_hit}")
 
 
-@pytest.mark.parametrize("kw", [dict(postpone_pct=20), dict(postpone_pct=40, prefetch=1, refill_idle=4,
+@pytest.mark.parametrize("kw", [dict(loop_mode=0, postpone_pct=20), dict(loop_mode=0, postpone_pct=40, prefetch=1, refill_idle=4,
                                                            refill_budget=8)])
 def test_cwbvh_postponing_within_contract(cw, orc, c0, c0_ref, kw):
     wl, lays = c0
@@ -154,6 +184,35 @@ def test_stats_counters_match_oracle(cw, orc, c0):
             assert np.array_equal(rc, gc), (k, rc, gc)
         else:
             assert np.all(gc >= rc) and np.all(gc <= 1.2 * rc), (k, rc, gc)
+
+
+@pytest.mark.parametrize("loop_mode", [0, 1, 2])
+def test_stats_counters_loop_modes(cw, orc, c0, loop_mode):
+    """The grouped and interleaved schedules keep each ray's order of node
+    visits and triangle tests, so their counters equal the oracle's.  Speculation tests a
+    node's triangles later than the oracle does, so the closest hit shrinks
+    tmax later and a few more nodes pass the slab test; the work stays close."""
+    wl, lays = c0
+    rays = wl.rays[:200000]
+    lay = lays["cwbvh"]
+    say(f"counters loop_mode={loop_mode}")
+    _, rc = oracle_trace(orc, lay, rays, counters=True)
+    cfg = cw.default_trace_config(cw.KIND_CWBVH, loop_mode=loop_mode)
+    _, gc = cw.Scene(lay).trace_numpy(rays, 0, cfg, with_stats=True)
+    if loop_mode in (0, 1):
+        assert np.array_equal(rc, gc), (rc, gc)
+    else:
+        assert np.all(gc >= 0.9 * rc) and np.all(gc <= 1.5 * rc), (rc, gc)
+
+
+def test_invalid_loop_configs_rejected(cw, c0):
+    wl, lays = c0
+    sc = cw.Scene(lays["cwbvh"])
+    for kw in (dict(loop_mode=3), dict(loop_mode=-1), dict(loop_mode=2, postpone_pct=20)):
+        cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw)
+        with pytest.raises(cw.CwbvhError) as e:
+            sc.trace_numpy(wl.rays[:64], 0, cfg)
+        assert e.value.status == 1, kw
 
 
 def test_prim_id_output(cw, c0, c0_ref):
