@@ -436,19 +436,29 @@ def main() -> int:
     nodes_c, tris_c = stats["cwbvh_closest"]
     bytes_c = algorithmic_bytes(1, n, nodes_c, tris_c)
     achieved = bytes_c / (ms_closest / 1e3) / 1e9
+    # HBM-side bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    # WRITE_SIZE calibrated on a 1 GiB copy) for this workload and config
     traffic = None
     if PROFILE_TRAFFIC.exists():
         try:
             t = json.loads(PROFILE_TRAFFIC.read_text())
-            if t.get("workload") == wl.name and t.get("rays") == n:
+            if t.get("workload") == wl.name and t.get("rays") == n and t.get("config") == cfg_cw.as_dict():
                 traffic = t.get("hbm_bytes_per_launch")
         except (ValueError, OSError):
             traffic = None
+    compulsory = 48 * n + s_cw.device_bytes()
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "kernel": "cwbvh_trace<closest> (persistent CWBVH, closest hit)",
             "algorithmic_bytes_per_launch": bytes_c, "avg_launch_ms": round(ms_closest, 4),
-            "per_ray": {"nodes": nodes_c / n, "triangles": tris_c / n, "bytes": bytes_c / n}}
+            "per_ray": {"nodes": nodes_c / n, "triangles": tris_c / n, "bytes": bytes_c / n},
+            # frac > 1 is possible: algorithmic node/triangle bytes are mostly
+            # served by L2; the measured HBM-side rate and the compulsory floor
+            # (rays + hits + scene once) bracket the real DRAM load
+            "hbm_side_gbs": round(traffic / (ms_closest / 1e3) / 1e9, 1) if traffic else None,
+            "compulsory_bytes_per_launch": compulsory,
+            "compulsory_gbs": round(compulsory / (ms_closest / 1e3) / 1e9, 1),
+            "limiter": "VALU issue (SQ_INSTS_VALU x 4 cycles / 1024 SIMDs ~ launch time)"}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -1,7 +1,7 @@
 """cwbvh_amd: MI355X (gfx950) ray traversal -- CWBVH and BVH2, closest and any hit.
 
 Python host side over the C ABI of ``libcwbvh_amd.so`` (include/cwbvh_amd.h).
-The library is built in-tree by ``python -m cwbvh_amd.build``; importing this
+The library is built in-tree by ``python cwbvh_amd/build.py``; importing this
 package without it raises ImportError -- there is no fallback path.
 
 Mirrors the reference's host surface:
@@ -86,6 +86,7 @@ class TraceConfig(C.Structure):
         ("work_slices", C.c_int32),
         ("persistent", C.c_int32),
         ("loop_mode", C.c_int32),
+        ("tri_repeat_pct", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -218,6 +219,7 @@ _SIGS = {
     "cwbvh_trace": (_I32, [_P, _P, _P, _I64, _U32, _P, _P, _P]),
     "cwbvh_trace_ordered": (_I32, [_P, _P, _P, _P, _I64, _U32, _P, _P, _P]),
     "cwbvh_stack_overflowed": (_I32, [_P, C.POINTER(_I32)]),
+    "cwbvh_loop_profile": (_I32, [_P, _P]),
     "cwbvh_scene_bounds": (_I32, [_P, _P, _P]),
     "cwbvh_morton_order": (_I32, [_P, _I64, _P, _P, _P, _P]),
     "cwbvh_batch_create": (_I32, [_P, _I32, _U32, _P, _I32, _PP]),
@@ -249,7 +251,7 @@ _SIGS = {
 
 def _load() -> C.CDLL:
     if not _LIB_PATH.exists():
-        raise ImportError(f"{_LIB_PATH} is missing; build it with `python -m cwbvh_amd.build`")
+        raise ImportError(f"{_LIB_PATH} is missing; build it with `python cwbvh_amd/build.py`")
     lib = C.CDLL(str(_LIB_PATH))
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)
@@ -852,6 +854,19 @@ def stack_overflowed(stream: Stream | None = None) -> bool:
     f = C.c_int32()
     check(lib.cwbvh_stack_overflowed(stream.handle if stream else None, C.byref(f)))
     return bool(f.value)
+
+
+def loop_profile(stream: Stream | None = None) -> dict:
+    """Loop-occupancy counters of the CWBVH STATS launches on `stream` since
+    the last call (cleared on read), with the lane utilization of each part."""
+    out = np.zeros(6, np.uint64)
+    check(lib.cwbvh_loop_profile(stream.handle if stream else None, out.ctypes.data))
+    names = ("iteration_slots", "active_lanes", "node_slots", "node_steps", "tri_slots", "tri_tests")
+    d = {k: int(v) for k, v in zip(names, out)}
+    for part, (u, s) in {"iteration": ("active_lanes", "iteration_slots"), "node": ("node_steps", "node_slots"),
+                         "tri": ("tri_tests", "tri_slots")}.items():
+        d[f"{part}_util"] = d[u] / d[s] if d[s] else 0.0
+    return d
 
 
 # ------------------------------------------------ reference entry points

@@ -3,7 +3,8 @@
 HIP sources (*.hip) go through hipcc with --offload-arch=gfx950; host C++
 (*.cpp) through g++.  Objects land in cwbvh_amd/_build/.  Usage:
 
-    python -m cwbvh_amd.build          # or: python cwbvh_amd/build.py
+    python cwbvh_amd/build.py   (works while the library is missing or stale;
+                                 `python -m cwbvh_amd.build` imports the package first)
 """
 from __future__ import annotations
 

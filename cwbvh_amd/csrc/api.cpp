@@ -425,6 +425,17 @@ int32_t cwbvh_trace_ordered(const cwbvh_scene* scene, const cwbvh_ray* d_rays, c
     return trace_impl(scene, d_rays, d_order, d_hits, n, flags, cfg, d_counters, stream);
 }
 
+int32_t cwbvh_loop_profile(void* stream, uint64_t* out6) {
+    CW_CHECK_ARG(out6 != nullptr, "out is NULL");
+    unsigned* c = nullptr;
+    int32_t rc = stream_counters((hipStream_t)stream, &c);
+    if (rc) return rc;
+    CW_HIP(hipMemcpyAsync(out6, c + kWorkProfile, 6 * sizeof(uint64_t), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    CW_HIP(hipMemsetAsync(c + kWorkProfile, 0, 6 * sizeof(uint64_t), (hipStream_t)stream));
+    CW_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return CWBVH_OK;
+}
+
 int32_t cwbvh_stack_overflowed(void* stream, int32_t* flag) {
     CW_CHECK_ARG(flag != nullptr, "flag is NULL");
     unsigned* c = nullptr;

@@ -232,6 +232,7 @@ int32_t cwbvh_batch_create(const cwbvh_batch_item* items, int32_t count, uint32_
         J.n = (int32_t)it.n;
         J.node_stride = it.scene->node_stride;
         J.chunk_begin = (uint32_t)chunks;
+        J.spec_limit = speculation_limit(it.scene->stack_need);
         chunks += ((uint64_t)it.n + c.fetch_chunk - 1) / c.fetch_chunk;
         if (chunks > 0xFFFFFFF0ull) {
             delete b;

@@ -45,7 +45,8 @@ constexpr int kWorkSlices = 8;
 constexpr int kWorkStride = 64;                       // uint32 per counter line
 constexpr int kWorkExited = kWorkSlices * kWorkStride;
 constexpr int kWorkOverflow = kWorkExited + kWorkStride;
-constexpr int kWorkWords = kWorkOverflow + kWorkStride;
+constexpr int kWorkProfile = kWorkOverflow + kWorkStride;   // uint64[6] loop-occupancy counters
+constexpr int kWorkWords = kWorkProfile + kWorkStride;
 int32_t stream_counters(hipStream_t stream, unsigned** out);
 
 struct TraceLaunch {
@@ -78,9 +79,17 @@ struct BatchJob {
     int32_t n;
     int32_t node_stride;
     uint32_t chunk_begin;   // first global chunk of this job
-    uint32_t pad;
+    int32_t spec_limit;     // speculative pushes allowed while the stack holds fewer entries
 };
 static_assert(sizeof(BatchJob) == 64, "BatchJob is 64 bytes");
+
+// Stack room a speculating lane may fill with parked triangle groups: the
+// node-group entries of a tree never exceed its stack_need, so speculation
+// below (capacity - stack_need) cannot overflow.  0 (no speculation) for a
+// tree of unknown depth.
+inline int32_t speculation_limit(int32_t stack_need) {
+    return stack_need > 0 && stack_need < CWBVH_STACK_CAPACITY ? CWBVH_STACK_CAPACITY - stack_need : 0;
+}
 
 struct BatchLaunch {
     const BatchJob* d_jobs;

@@ -157,7 +157,7 @@ int32_t cwbvh_server::launch(Slot& sl, const std::vector<Request*>& batch) {
         J.n = (int32_t)r->n;
         J.node_stride = sc->node_stride;
         J.chunk_begin = chunks;
-        J.pad = 0;
+        J.spec_limit = speculation_limit(sc->stack_need);
         chunks += (uint32_t)((r->n + chunk - 1) / chunk);
     }
     if (cfg.serial) {

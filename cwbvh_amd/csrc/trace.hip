@@ -35,6 +35,8 @@
 
 using namespace cwd;
 
+static_assert(kStackCapacity == CWBVH_STACK_CAPACITY, "device and public stack capacities agree");
+
 namespace {
 
 constexpr int kMaxBlock = 256;
@@ -42,6 +44,7 @@ constexpr int kWorkSlices = cw::kWorkSlices;
 constexpr int kWorkStride = cw::kWorkStride;
 constexpr int kWorkExited = cw::kWorkExited;
 constexpr int kWorkOverflow = cw::kWorkOverflow;
+constexpr int kWorkProfile = cw::kWorkProfile;
 
 struct CwbvhParams {
     const float4* __restrict__ nodes;
@@ -65,6 +68,8 @@ struct CwbvhParams {
     int postpone_pct;
     int prim_id;
     int total_waves;
+    int spec_limit;                 // single scene: speculation_limit of its tree
+    int tri_repeat_pct;
 };
 
 // Quantized slab test of 4 children (one meta dword, one byte per child).
@@ -109,7 +114,16 @@ __device__ __forceinline__ unsigned child_hits4(unsigned meta4, unsigned lox, un
 //  kLoopSpeculate: as kLoopInterleave, and a lane that holds triangles still
 //    visits a node in the same iteration; the new node's triangle group is
 //    pushed on the stack and tested later, so node visits keep every lane
-//    with inner children busy.
+//    with inner children busy.  With postpone_pct > 0 the wave also holds
+//    back its triangle step until that share of its lanes have triangles (or
+//    no lane can visit a node), so the step runs with more lanes on.
+//  In both one-triangle schedules, tri_repeat_pct > 0 repeats the triangle
+//  step within the iteration while at least that share of the active lanes
+//  still hold triangles: triangle-dense rays get the grouped schedule's
+//  triangle loop, sparse ones keep single steps.
+//    A lane speculates only while its stack holds fewer than spec_limit
+//    entries (capacity minus the tree's stack need), so the parked groups
+//    never push the stack past its capacity.
 constexpr int kLoopGroup = 0;
 constexpr int kLoopInterleave = 1;
 constexpr int kLoopSpeculate = 2;
@@ -129,6 +143,7 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
     float4* hits = P.hits;
     const int* prim_map = P.prim_map;
     int stride = P.stride;
+    int spec_limit = P.spec_limit;
     float3 o = make_float3(0.f, 0.f, 0.f), d = o;
     float tmin = 0.f, tmax = 0.f, idx = 0.f, idy = 0.f, idz = 0.f;
     unsigned octinv = 0, octinv4 = 0;
@@ -137,6 +152,17 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
     int hit_addr = -1;
     float hu = 0.f, hv = 0.f;
     unsigned long long st_nodes = 0, st_tris = 0;
+    // STATS only: {slots, uses} of the loop iteration, the node code and the
+    // triangle code; a slot is one lane of a wave instruction stream that ran
+    // the code (64 per wave pass), a use is a lane that needed it
+    unsigned long long pr_it = 0, pr_act = 0, pr_nd = 0, pr_ndu = 0, pr_tr = 0, pr_tru = 0;
+    auto prof = [&](bool used, unsigned long long& slots, unsigned long long& uses) {
+        if (STATS) {
+            const unsigned long long act = __ballot(1);
+            if (__any(used) && lane == __ffsll((long long)act) - 1) slots += 64;
+            uses += used ? 1 : 0;
+        }
+    };
     bool have_pf = false;
     float4 pf0, pf1, pf2, pf3, pf4;
     // A lane is busy from loading a ray until writing its hit.  (Deriving
@@ -242,6 +268,7 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
                 hits = (float4*)J.hits;
                 prim_map = J.prim_idx;
                 stride = J.node_stride;
+                spec_limit = J.spec_limit;
                 rays = (const Ray4*)J.rays;
                 order = J.order;
             }
@@ -359,7 +386,9 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
         int lost = 0;
         while (true) {
             bool finished = false;
+            prof(true, pr_it, pr_act);
             if constexpr (LOOP == kLoopGroup) {
+                prof(ng.y > 0x00FFFFFFu, pr_nd, pr_ndu);
                 if (ng.y > 0x00FFFFFFu) {
                     tg = visit_node();
                 } else {
@@ -377,6 +406,7 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
                             break;
                         }
                     }
+                    prof(true, pr_tr, pr_tru);
                     if (test_triangle()) {
                         finished = true;
                         break;
@@ -393,17 +423,36 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
             } else {
                 // node step: lanes without triangles (and, speculating, every
                 // lane with inner children left) visit one node
-                const bool node_step = (LOOP == kLoopSpeculate || tg.y == 0u) && ng.y > 0x00FFFFFFu;
+                const bool spec_ok = LOOP == kLoopSpeculate && sp < spec_limit;
+                const bool node_step = (spec_ok || tg.y == 0u) && ng.y > 0x00FFFFFFu;
+                prof(node_step, pr_nd, pr_ndu);
                 if (node_step) {
                     const uint2 ntg = visit_node();
-                    if (LOOP == kLoopSpeculate && tg.y != 0u) {
+                    if (tg.y != 0u) {
                         if (ntg.y != 0u) push(ntg);
                     } else {
                         tg = ntg;
                     }
                 }
-                // triangle step: one test
-                if (tg.y != 0u) finished = test_triangle();
+                // triangle step: one test (speculating with postpone_pct, only
+                // once enough lanes hold triangles or a lane holds nothing else)
+                bool run_tri = true;
+                if (LOOP == kLoopSpeculate && P.postpone_pct > 0) {
+                    const bool can_visit = ng.y > 0x00FFFFFFu && (tg.y == 0u || sp < spec_limit);
+                    const int holders = __popcll(__ballot(tg.y != 0u));
+                    run_tri = !__any(can_visit) || holders * 100 >= P.postpone_pct * __popcll(__ballot(1));
+                }
+                prof(run_tri && tg.y != 0u, pr_tr, pr_tru);
+                if (run_tri && tg.y != 0u) finished = test_triangle();
+                if (run_tri && P.tri_repeat_pct > 0) {
+                    while (true) {
+                        const bool has_t = !finished && tg.y != 0u;
+                        const int holders = __popcll(__ballot(has_t));
+                        if (holders * 100 < P.tri_repeat_pct * __popcll(__ballot(1))) break;
+                        prof(has_t, pr_tr, pr_tru);
+                        if (has_t) finished = test_triangle();
+                    }
+                }
                 // out of inner children: take the next group off the stack
                 if (!finished && ng.y <= 0x00FFFFFFu) {
                     if (tg.y == 0u) {
@@ -446,6 +495,13 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
     if (STATS) {
         atomicAdd(&P.stats[0], st_nodes);
         atomicAdd(&P.stats[1], st_tris);
+        unsigned long long* pr = reinterpret_cast<unsigned long long*>(P.work + kWorkProfile);
+        atomicAdd(&pr[0], pr_it);
+        atomicAdd(&pr[1], pr_act);
+        atomicAdd(&pr[2], pr_nd);
+        atomicAdd(&pr[3], pr_ndu);
+        atomicAdd(&pr[4], pr_tr);
+        atomicAdd(&pr[5], pr_tru);
     }
     // last wave out resets the fetch counters for the next launch on this stream
     // (one-ray-per-thread launches claim nothing, so they skip the exit count)
@@ -688,12 +744,17 @@ int32_t validate_config(int32_t kind, const cwbvh_trace_config& c) {
             c.persistent < 0 || c.persistent > 1 ||
             c.refill_idle < 0 || c.refill_idle > 64 || c.postpone_pct < 0 || c.postpone_pct > 100 ||
             c.prefetch < 0 || c.prefetch > 1 || c.work_slices < 1 || c.work_slices > kWorkSlices ||
-            c.loop_mode < kLoopGroup || c.loop_mode > kLoopSpeculate) {
+            c.loop_mode < kLoopGroup || c.loop_mode > kLoopSpeculate ||
+            c.tri_repeat_pct < 0 || c.tri_repeat_pct > 100) {
             set_error("invalid CWBVH trace configuration");
             return CWBVH_ERR_INVALID_ARGUMENT;
         }
-        if (c.postpone_pct > 0 && c.loop_mode != kLoopGroup) {
-            set_error("postpone_pct applies to loop_mode 0 (the other schedules test one triangle per iteration)");
+        if (c.postpone_pct > 0 && c.loop_mode == kLoopInterleave) {
+            set_error("postpone_pct applies to loop_mode 0 and 2");
+            return CWBVH_ERR_INVALID_ARGUMENT;
+        }
+        if (c.tri_repeat_pct > 0 && c.loop_mode == kLoopGroup) {
+            set_error("tri_repeat_pct applies to loop_mode 1 and 2");
             return CWBVH_ERR_INVALID_ARGUMENT;
         }
     }
@@ -723,6 +784,8 @@ int32_t launch_cwbvh(const TraceLaunch& L) {
     p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
     p.slices = L.cfg.work_slices;
     p.persistent = L.cfg.persistent;
+    p.spec_limit = cw::speculation_limit(s->stack_need);
+    p.tri_repeat_pct = L.cfg.tri_repeat_pct;
     return dispatch_cwbvh<false>(p, L.flags, L.cfg, L.stream);
 }
 
@@ -745,6 +808,7 @@ int32_t launch_cwbvh_batch(const BatchLaunch& L) {
     p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
     p.slices = 1;
     p.persistent = 1;
+    p.tri_repeat_pct = L.cfg.tri_repeat_pct;
     return dispatch_cwbvh<true>(p, L.flags, L.cfg, L.stream);
 }
 

@@ -261,7 +261,9 @@ typedef struct cwbvh_trace_config {
     int32_t fetch_chunk;     /* rays claimed per wave-level atomic (CWBVH; 0 = automatic) */
     int32_t refill_idle;     /* leave the inner loop once idle lanes exceed this ... */
     int32_t refill_budget;   /* ... for this many accumulated lane-iterations */
-    int32_t postpone_pct;    /* postpone a triangle group below this % of active lanes (0 = off) */
+    int32_t postpone_pct;    /* 0 = off; loop_mode 0: postpone a triangle group when fewer than this %
+                                of the lanes are active; loop_mode 2: hold triangle tests back until
+                                this % of the active lanes have triangles; not for loop_mode 1 */
     int32_t prefetch;        /* 0 or 1: issue the next node's loads before the triangle loop */
     int32_t work_slices;     /* 1..8 contiguous ray ranges, one drained first per XCD (CWBVH) */
     int32_t persistent;      /* CWBVH: 1 = persistent grid with work fetching, 0 = one ray per thread */
@@ -269,6 +271,8 @@ typedef struct cwbvh_trace_config {
                                 per iteration (reference order), 1 = node visit or one triangle test
                                 per iteration, 2 = as 1, and lanes holding triangles also visit a
                                 node, parking its triangle group on the stack */
+    int32_t tri_repeat_pct;  /* loop_mode 1 and 2: repeat the triangle test within an iteration while
+                                at least this % of the active lanes hold triangles (0 = one test) */
 } cwbvh_trace_config;
 
 int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
@@ -288,6 +292,14 @@ int32_t cwbvh_trace_ordered(const cwbvh_scene* scene, const cwbvh_ray* d_rays,
 /* 1 when a traversal on this stream overflowed its stack since the last
  * call (the flag is cleared); the affected rays may have missed hits. */
 int32_t cwbvh_stack_overflowed(void* stream, int32_t* flag);
+
+/* Diagnostic: loop-occupancy counters accumulated by CWBVH launches with
+ * CWBVH_TRACE_STATS on this stream since the last call (then cleared):
+ * out[6] = {iteration slots, active lane-iterations, node-code slots,
+ * node steps, triangle-code slots, triangle tests}.  A slot is one lane of a
+ * wave pass through that code (64 per pass), so uses / slots is the SIMD lane
+ * utilization of each part of the inner loop. */
+int32_t cwbvh_loop_profile(void* stream, uint64_t* out6);
 
 /* Scene bounds (lo[3], hi[3]) as stored at creation (wrapped scenes: unit cube). */
 int32_t cwbvh_scene_bounds(const cwbvh_scene* scene, float* lo3, float* hi3);

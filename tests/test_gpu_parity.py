@@ -91,6 +91,13 @@ CWBVH_VARIANTS = [
     dict(loop_mode=2, prefetch=1, lds_depth=1, fetch_chunk=1),
     dict(loop_mode=2, block_size=256, refill_idle=0, refill_budget=1, work_slices=8),
     dict(loop_mode=2, persistent=0, block_size=128, lds_depth=64),
+    dict(loop_mode=2, postpone_pct=50),
+    dict(loop_mode=2, postpone_pct=90, lds_depth=4, block_size=256),
+    dict(loop_mode=2, postpone_pct=100, block_size=64, fetch_chunk=3, prefetch=1),
+    dict(loop_mode=1, tri_repeat_pct=50),
+    dict(loop_mode=1, tri_repeat_pct=1, lds_depth=2, persistent=0),
+    dict(loop_mode=2, tri_repeat_pct=70, postpone_pct=40),
+    dict(loop_mode=2, tri_repeat_pct=100, block_size=256, refill_idle=0, refill_budget=1),
 ]
 
 
@@ -124,6 +131,7 @@ def test_cwbvh_config_variants_bit_identical(cw, orc, c0, c0_ref, variant):
             assert_any_hit_valid(orc, lays["cwbvh"], wl.rays, g, c0_ref[("cwbvh", True)], what)
         else:
             assert_bitwise(g, c0_ref[("cwbvh", any_hit)], what)
+        assert not cw.stack_overflowed(), what
 
 
 @pytest.mark.parametrize("kw", [dict(block_size=64, lds_depth=1), dict(block_size=256, lds_depth=8),
@@ -186,8 +194,9 @@ def test_stats_counters_match_oracle(cw, orc, c0):
             assert np.all(gc >= rc) and np.all(gc <= 1.2 * rc), (k, rc, gc)
 
 
-@pytest.mark.parametrize("loop_mode", [0, 1, 2])
-def test_stats_counters_loop_modes(cw, orc, c0, loop_mode):
+@pytest.mark.parametrize("loop_mode,postpone,repeat", [(0, 0, 0), (1, 0, 0), (1, 0, 50), (2, 0, 0), (2, 60, 0),
+                                                        (2, 60, 60)])
+def test_stats_counters_loop_modes(cw, orc, c0, loop_mode, postpone, repeat):
     """The grouped and interleaved schedules keep each ray's order of node
     visits and triangle tests, so their counters equal the oracle's.  Speculation tests a
     node's triangles later than the oracle does, so the closest hit shrinks
@@ -197,7 +206,7 @@ def test_stats_counters_loop_modes(cw, orc, c0, loop_mode):
     lay = lays["cwbvh"]
     say(f"counters loop_mode={loop_mode}")
     _, rc = oracle_trace(orc, lay, rays, counters=True)
-    cfg = cw.default_trace_config(cw.KIND_CWBVH, loop_mode=loop_mode)
+    cfg = cw.default_trace_config(cw.KIND_CWBVH, loop_mode=loop_mode, postpone_pct=postpone, tri_repeat_pct=repeat)
     _, gc = cw.Scene(lay).trace_numpy(rays, 0, cfg, with_stats=True)
     if loop_mode in (0, 1):
         assert np.array_equal(rc, gc), (rc, gc)
@@ -205,10 +214,31 @@ def test_stats_counters_loop_modes(cw, orc, c0, loop_mode):
         assert np.all(gc >= 0.9 * rc) and np.all(gc <= 1.5 * rc), (rc, gc)
 
 
+@pytest.mark.parametrize("kw", [dict(loop_mode=0), dict(loop_mode=1), dict(loop_mode=1, tri_repeat_pct=50),
+                                dict(loop_mode=2, postpone_pct=60)])
+def test_loop_profile_counters(cw, c0, kw):
+    """The occupancy counters of a STATS launch agree with its work counters:
+    one node step per visited node, one triangle use per test, and no part of
+    the loop uses more lanes than it ran with."""
+    wl, lays = c0
+    say(f"loop profile {kw}")
+    cw.loop_profile()  # clear
+    cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw)
+    _, gc = cw.Scene(lays["cwbvh"]).trace_numpy(wl.rays[:300000], 0, cfg, with_stats=True)
+    p = cw.loop_profile()
+    say(f"  {p}")
+    assert p["node_steps"] == gc[0] and p["tri_tests"] == gc[1]
+    for part in ("iteration", "node", "tri"):
+        assert 0 < p[f"{part}_util"] <= 1.0
+    assert p["node_slots"] <= p["iteration_slots"] and p["active_lanes"] <= p["iteration_slots"]
+    assert cw.loop_profile()["iteration_slots"] == 0  # cleared on read
+
+
 def test_invalid_loop_configs_rejected(cw, c0):
     wl, lays = c0
     sc = cw.Scene(lays["cwbvh"])
-    for kw in (dict(loop_mode=3), dict(loop_mode=-1), dict(loop_mode=2, postpone_pct=20)):
+    for kw in (dict(loop_mode=3), dict(loop_mode=-1), dict(loop_mode=1, postpone_pct=20),
+               dict(loop_mode=0, tri_repeat_pct=50), dict(loop_mode=1, tri_repeat_pct=101)):
         cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw)
         with pytest.raises(cw.CwbvhError) as e:
             sc.trace_numpy(wl.rays[:64], 0, cfg)
