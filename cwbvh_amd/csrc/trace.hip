@@ -60,7 +60,7 @@ struct CwbvhParams {
     int slices;                     // single scene: contiguous ray ranges, one per XCD
     int persistent;                 // 0: one ray per thread, no refill
     int num_jobs;
-    int stride;                     // float4 per node
+    int stride;                     // float4 per node: 5 or 8
     int lds_depth;
     int chunk;
     int idle_lanes;
@@ -72,6 +72,17 @@ struct CwbvhParams {
     int tri_repeat_pct;
 };
 
+__device__ __forceinline__ const float4* node_at(const float4* nodes, unsigned i, unsigned stride_sh) {
+    return nodes + ((i << 2) + (i << stride_sh));
+}
+
+// Keeps the compiler from folding shift-and-subtract into a quarter-rate
+// v_mul_lo_u32 (it emits nothing).
+__device__ __forceinline__ unsigned opaque(unsigned x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 // Quantized slab test of 4 children (one meta dword, one byte per child).
 __device__ __forceinline__ unsigned child_hits4(unsigned meta4, unsigned lox, unsigned loy, unsigned loz,
                                                 unsigned hix, unsigned hiy, unsigned hiz, float adjx, float adjy,
@@ -79,7 +90,7 @@ __device__ __forceinline__ unsigned child_hits4(unsigned meta4, unsigned lox, un
                                                 float tmax, unsigned octinv4) {
     const unsigned is_inner4 = (meta4 & (meta4 << 1)) & 0x10101010u;
     const unsigned ones = is_inner4 >> 4;                 // 0x01 per inner child byte
-    const unsigned inner_mask4 = (ones << 8) - ones;      // 0xFF per inner child byte
+    const unsigned inner_mask4 = opaque(ones << 3) - ones;  // 0x07 per inner child byte (octinv < 8)
     const unsigned bit_index4 = (meta4 ^ (octinv4 & inner_mask4)) & 0x1F1F1F1Fu;
     const unsigned child_bits4 = (meta4 >> 5) & 0x07070707u;
     unsigned hitmask = 0;
@@ -142,7 +153,9 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
     const float4* tris = P.tris;
     float4* hits = P.hits;
     const int* prim_map = P.prim_map;
-    int stride = P.stride;
+    // node i starts at float4 (i << 2) + (i << stride_sh): stride 5 or 8
+    // without a quarter-rate integer multiply
+    unsigned stride_sh = P.stride == 8 ? 2u : 0u;
     int spec_limit = P.spec_limit;
     float3 o = make_float3(0.f, 0.f, 0.f), d = o;
     float tmin = 0.f, tmax = 0.f, idx = 0.f, idy = 0.f, idz = 0.f;
@@ -267,7 +280,7 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
                 tris = (const float4*)J.woop;
                 hits = (float4*)J.hits;
                 prim_map = J.prim_idx;
-                stride = J.node_stride;
+                stride_sh = J.node_stride == 8 ? 2u : 0u;
                 spec_limit = J.spec_limit;
                 rays = (const Ray4*)J.rays;
                 order = J.order;
@@ -321,7 +334,7 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
             } else {
                 const unsigned slot = (unsigned)(bit - 24) ^ octinv;
                 const unsigned rel = (unsigned)__popc(imask & ~(0xFFFFFFFFu << slot));
-                const float4* np = nodes + (size_t)(base + rel) * (unsigned)stride;
+                const float4* np = node_at(nodes, base + rel, stride_sh);
                 n0 = np[0]; n1 = np[1]; n2 = np[2]; n3 = np[3]; n4 = np[4];
             }
             if (STATS) ++st_nodes;
@@ -359,7 +372,7 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
                 const int nb = 31 - __clz((int)ng.y);
                 const unsigned nslot = (unsigned)(nb - 24) ^ octinv;
                 const unsigned nrel = (unsigned)__popc(ng.y & ~(0xFFFFFFFFu << nslot));
-                const float4* np = nodes + (size_t)(ng.x + nrel) * (unsigned)stride;
+                const float4* np = node_at(nodes, ng.x + nrel, stride_sh);
                 pf0 = np[0]; pf1 = np[1]; pf2 = np[2]; pf3 = np[3]; pf4 = np[4];
                 have_pf = true;
             }
