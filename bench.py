@@ -113,11 +113,11 @@ def bench_c2(args, cw, W, dev) -> int:
 
     def plain():
         for sc, r, h, n in items:
-            sc.trace(r, h, n, 0, cfg0, None, st)
+            sc.trace(r, h, n, 0, None, None, st)
 
     def plain_sync():
         for sc, r, h, n in items:
-            sc.trace(r, h, n, 0, cfg0, None, st)
+            sc.trace(r, h, n, 0, None, None, st)
             st.synchronize()
 
     def timed(fn, steps, warmup):
@@ -197,6 +197,7 @@ def bench_c3(args, cw, W, dev) -> int:
     log(f"C3: {mesh.num_triangles} tris, {n} rays, gen {gen_s:.1f} s, build {build_s:.1f} s, "
         f"cwbvh {cwb.stats()}, bvh2 {bvh2.stats()}")
     s_cw, s_b2 = cw.Scene(cwb), cw.Scene(bvh2)
+    s_cw8 = cw.Scene(cwb.restride(8))  # 128 B re-lay: one cache line per node
     lo, hi = s_cw.bounds()
     del bvh2
     dr = cw.DeviceBuffer.from_numpy(rays)
@@ -209,7 +210,7 @@ def bench_c3(args, cw, W, dev) -> int:
     e1.record()
     e1.synchronize()
     sort_ms = e0.elapsed_ms(e1)
-    cfg_cw = cw.default_trace_config(cw.KIND_CWBVH)
+    cfg_cw = s_cw.default_config()
     cfg_b2 = cw.default_trace_config(cw.KIND_BVH2)
 
     def t(sc, cfg, order):
@@ -223,6 +224,17 @@ def bench_c3(args, cw, W, dev) -> int:
             ms = t(sc, cfg, order)
             res[f"{name}_{oname}"] = {"ms": round(ms, 3), "mrays_per_s": round(n / ms / 1e3, 1)}
             log(f"{name} {oname}: {ms:.2f} ms {n / ms / 1e3:.0f} Mrays/s")
+    # node layout x inner-loop schedule: this scene is far larger than L2 and
+    # the Infinity Cache, so line-aligned nodes and access order matter
+    layouts = []
+    for lname, sc in (("node80", s_cw), ("node128", s_cw8)):
+        for mode in (0, 1, 2):
+            cfg = cw.default_trace_config(cw.KIND_CWBVH, loop_mode=mode)
+            for oname, order in (("shuffled", None), ("morton", do.ptr)):
+                ms = t(sc, cfg, order)
+                layouts.append({"nodes": lname, "loop_mode": mode, "order": oname, "ms": round(ms, 3),
+                                "mrays_per_s": round(n / ms / 1e3, 1)})
+                log(f"cwbvh {lname} loop_mode {mode} {oname}: {ms:.2f} ms {n / ms / 1e3:.0f} Mrays/s")
     d_cnt = cw.DeviceBuffer(16)
     d_cnt.zero()
     s_cw.trace(dr.ptr, dh.ptr, n, cw.TRACE_STATS, cfg_cw, d_cnt.ptr)
@@ -249,6 +261,8 @@ def bench_c3(args, cw, W, dev) -> int:
                                f"rays from surface points, tmax=inf, closest hit, shuffled order",
                    "triangles": mesh.num_triangles, "rays": n, "seed": W.SEEDS["C3"]},
         "results": res,
+        "cwbvh_layout_schedule": layouts,
+        "best_cwbvh_shuffled": min((r for r in layouts if r["order"] == "shuffled"), key=lambda r: r["ms"]),
         "morton_sort_ms": round(sort_ms, 3),
         "cwbvh_vs_bvh2": {"shuffled": round(res["bvh2_shuffled"]["ms"] / res["cwbvh_shuffled"]["ms"], 3),
                           "morton": round(res["bvh2_morton"]["ms"] / res["cwbvh_morton"]["ms"], 3)},
@@ -404,7 +418,7 @@ def main() -> int:
     d_rays = cw.DeviceBuffer.from_numpy(rays)
     d_hits = cw.DeviceBuffer(16 * n)
     d_cnt = cw.DeviceBuffer(16)
-    cfg_cw = cw.default_trace_config(cw.KIND_CWBVH)
+    cfg_cw = s_cw.default_config()
     cfg_b2 = cw.default_trace_config(cw.KIND_BVH2)
 
     # algorithmic traffic (counting pass, outside the timed region)

@@ -86,7 +86,6 @@ class TraceConfig(C.Structure):
         ("work_slices", C.c_int32),
         ("persistent", C.c_int32),
         ("loop_mode", C.c_int32),
-        ("tri_repeat_pct", C.c_int32),
     ]
 
     def as_dict(self) -> dict:
@@ -220,6 +219,7 @@ _SIGS = {
     "cwbvh_trace_ordered": (_I32, [_P, _P, _P, _P, _I64, _U32, _P, _P, _P]),
     "cwbvh_stack_overflowed": (_I32, [_P, C.POINTER(_I32)]),
     "cwbvh_loop_profile": (_I32, [_P, _P]),
+    "cwbvh_trace_config_for_scene": (_I32, [_P, _P]),
     "cwbvh_scene_bounds": (_I32, [_P, _P, _P]),
     "cwbvh_morton_order": (_I32, [_P, _I64, _P, _P, _P, _P]),
     "cwbvh_batch_create": (_I32, [_P, _I32, _U32, _P, _I32, _PP]),
@@ -650,6 +650,20 @@ class Scene:
         check(lib.cwbvh_scene_create(layout.handle, stream.handle if stream else None, C.byref(s)))
         self._s = s
         self.kind = layout.kind
+        self._keep = ()
+
+    @classmethod
+    def wrap(cls, kind: int, node_stride: int, d_nodes: int, d_woop: int, d_prim_idx: int | None = None,
+             stack_need: int = 0, keep=()) -> "Scene":
+        """Non-owning scene over caller device buffers (cwbvh_scene_wrap_device);
+        `keep` holds the buffers alive as long as the scene."""
+        s = C.c_void_p()
+        check(lib.cwbvh_scene_wrap_device(kind, node_stride, d_nodes, d_woop, d_prim_idx, stack_need, C.byref(s)))
+        obj = cls.__new__(cls)
+        obj._s = s
+        obj.kind = kind
+        obj._keep = tuple(keep)
+        return obj
 
     @property
     def handle(self) -> C.c_void_p:
@@ -665,6 +679,12 @@ class Scene:
         hi = np.zeros(3, np.float32)
         check(lib.cwbvh_scene_bounds(self._s, _ptr(lo), _ptr(hi)))
         return lo, hi
+
+    def default_config(self, **overrides) -> TraceConfig:
+        """The configuration a trace without cfg uses for this scene."""
+        c = TraceConfig()
+        check(lib.cwbvh_trace_config_for_scene(self._s, C.byref(c)))
+        return _set_fields(c, overrides)
 
     def trace(self, d_rays: int, d_hits: int, n: int, flags: int = TRACE_CLOSEST, cfg: TraceConfig | None = None,
               d_counters: int | None = None, stream: Stream | None = None, d_order: int | None = None) -> None:

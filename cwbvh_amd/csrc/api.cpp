@@ -324,7 +324,11 @@ int32_t cwbvh_scene_wrap_device(int32_t kind, int32_t node_stride, const void* d
     if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)d_woop) == hipSuccess && size > 0) {
         size_t off = (size_t)((const char*)d_woop - (const char*)base);
         s->woop_f4 = (int64_t)((size - off) / sizeof(cwbvh_float4));
+        s->bytes += (int64_t)(size - off);
     }
+    // node bytes: from the node pointer to the end of its allocation (an upper bound)
+    if (hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)d_nodes) == hipSuccess && size > 0)
+        s->bytes += (int64_t)(size - (size_t)((const char*)d_nodes - (const char*)base));
     hipGetLastError();
     *out = s;
     return CWBVH_OK;
@@ -359,9 +363,10 @@ int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* c) {
     CW_CHECK_ARG(kind == CWBVH_KIND_BVH2 || kind == CWBVH_KIND_CWBVH, "bad kind");
     std::memset(c, 0, sizeof(*c));
     if (kind == CWBVH_KIND_CWBVH) {
-        // measured on MI355X (C1, 33M AO rays): 10 LDS entries cover the
-        // stack of trees up to depth 9 at full occupancy; ~256-ray claims
-        // keep the fetch atomics off the critical path; the interleaved
+        // measured on MI355X (C1, 33M AO rays; C0, 1M): 10 LDS entries cover
+        // the stack of trees up to depth 9 at full occupancy; 128..512-ray
+        // claims spread over 8 counters keep the fetch atomics off the
+        // critical path (C1 +2%, C0 +46% over one counter); the interleaved
         // schedule is 1.36x the grouped one and keeps the reference's
         // per-ray order (speculation adds under 3%)
         c->block_size = 128;
@@ -372,14 +377,27 @@ int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* c) {
         c->refill_budget = 16;
         c->postpone_pct = 0;
         c->prefetch = 0;
-        c->work_slices = 1;
-        c->persistent = 1;
+        c->work_slices = 8;
+        c->persistent = 2;    // automatic by ray count
         c->loop_mode = 1;
     } else {
         c->block_size = 256;
         c->lds_depth = 8;
         c->work_slices = 1;
     }
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_config* c) {
+    CW_CHECK_ARG(scene != nullptr && c != nullptr, "NULL argument");
+    int32_t rc = cwbvh_trace_config_default(scene->kind, c);
+    if (rc) return rc;
+    // measured on MI355X: a tree that fits the 256 MB Infinity Cache is
+    // VALU-bound and gains from the interleaved schedule (C1: 1.36x); a
+    // larger one is bound by HBM traffic, which the grouped schedule keeps
+    // lower by testing a leaf's triangles right after fetching the node
+    // (C3, 1.2 GB: 1.10x over interleaved)
+    if (scene->kind == CWBVH_KIND_CWBVH && scene->bytes > kCacheResidentBytes) c->loop_mode = 0;
     return CWBVH_OK;
 }
 
@@ -394,7 +412,7 @@ static int32_t trace_impl(const cwbvh_scene* scene, const cwbvh_ray* d_rays, con
     CW_CHECK_ARG(!(flags & CWBVH_TRACE_PRIM_ID) || scene->prim_idx != nullptr, "PRIM_ID needs a prim index array");
     cwbvh_trace_config c;
     if (cfg) c = *cfg;
-    else cwbvh_trace_config_default(scene->kind, &c);
+    else cwbvh_trace_config_for_scene(scene, &c);
     int32_t rc = validate_config(scene->kind, c);
     if (rc) return rc;
     if (n == 0) return CWBVH_OK;

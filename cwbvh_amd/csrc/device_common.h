@@ -88,8 +88,7 @@ __device__ __forceinline__ CWD_LDS T* lds_ptr(T* p) {
 
 template <class T, int CAP>
 __device__ __forceinline__ void stack_put(CWD_LDS T* lds, T (&spill)[CAP], const StackIndex& s, int sp, T v) {
-    // 24-bit multiply (full rate): sp < 64 and the block is at most 256 lanes
-    if (sp < s.depth) lds[__umul24((unsigned)sp, (unsigned)s.stride) + (unsigned)s.col] = v;
+    if (sp < s.depth) lds[sp * s.stride + s.col] = v;
     else spill[sp - s.depth] = v;
 }
 
@@ -97,7 +96,7 @@ __device__ __forceinline__ void stack_put(CWD_LDS T* lds, T (&spill)[CAP], const
 // cannot be merged into one generic (flat) load through a selected pointer.
 template <class T, int CAP>
 __device__ __forceinline__ T stack_get(const CWD_LDS T* lds, const T (&spill)[CAP], const StackIndex& s, int sp) {
-    if (sp < s.depth) return lds[__umul24((unsigned)sp, (unsigned)s.stride) + (unsigned)s.col];
+    if (sp < s.depth) return lds[sp * s.stride + s.col];
     return spill[sp - s.depth];
 }
 

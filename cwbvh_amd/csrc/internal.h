@@ -41,6 +41,13 @@ int32_t hip_fail(hipError_t e, const char* what);
 // graph replays cleanly).  The overflow flag stays set until read.
 // Each counter sits on its own 256-byte line so that the slices' atomics are
 // served by different memory channels instead of serializing on one line.
+// Scenes larger than the MI355X Infinity Cache (256 MB) are traversed in the
+// HBM-bound regime (see cwbvh_trace_config_for_scene).
+constexpr int64_t kCacheResidentBytes = 256ll << 20;
+
+// cwbvh_trace_config.persistent == 2: persistent grid from this many rays up.
+constexpr int64_t kAutoPersistentRays = 1ll << 19;
+
 constexpr int kWorkSlices = 8;
 constexpr int kWorkStride = 64;                       // uint32 per counter line
 constexpr int kWorkExited = kWorkSlices * kWorkStride;

@@ -68,19 +68,12 @@ struct CwbvhParams {
     int postpone_pct;
     int prim_id;
     int total_waves;
+    int total_blocks;
     int spec_limit;                 // single scene: speculation_limit of its tree
-    int tri_repeat_pct;
 };
 
 __device__ __forceinline__ const float4* node_at(const float4* nodes, unsigned i, unsigned stride_sh) {
     return nodes + ((i << 2) + (i << stride_sh));
-}
-
-// Keeps the compiler from folding shift-and-subtract into a quarter-rate
-// v_mul_lo_u32 (it emits nothing).
-__device__ __forceinline__ unsigned opaque(unsigned x) {
-    asm volatile("" : "+v"(x));
-    return x;
 }
 
 // Quantized slab test of 4 children (one meta dword, one byte per child).
@@ -90,7 +83,7 @@ __device__ __forceinline__ unsigned child_hits4(unsigned meta4, unsigned lox, un
                                                 float tmax, unsigned octinv4) {
     const unsigned is_inner4 = (meta4 & (meta4 << 1)) & 0x10101010u;
     const unsigned ones = is_inner4 >> 4;                 // 0x01 per inner child byte
-    const unsigned inner_mask4 = opaque(ones << 3) - ones;  // 0x07 per inner child byte (octinv < 8)
+    const unsigned inner_mask4 = (ones << 8) - ones;      // 0xFF per inner child byte
     const unsigned bit_index4 = (meta4 ^ (octinv4 & inner_mask4)) & 0x1F1F1F1Fu;
     const unsigned child_bits4 = (meta4 >> 5) & 0x07070707u;
     unsigned hitmask = 0;
@@ -128,10 +121,6 @@ __device__ __forceinline__ unsigned child_hits4(unsigned meta4, unsigned lox, un
 //    with inner children busy.  With postpone_pct > 0 the wave also holds
 //    back its triangle step until that share of its lanes have triangles (or
 //    no lane can visit a node), so the step runs with more lanes on.
-//  In both one-triangle schedules, tri_repeat_pct > 0 repeats the triangle
-//  step within the iteration while at least that share of the active lanes
-//  still hold triangles: triangle-dense rays get the grouped schedule's
-//  triangle loop, sparse ones keep single steps.
 //    A lane speculates only while its stack holds fewer than spec_limit
 //    entries (capacity minus the tree's stack need), so the parked groups
 //    never push the stack past its capacity.
@@ -139,8 +128,13 @@ constexpr int kLoopGroup = 0;
 constexpr int kLoopInterleave = 1;
 constexpr int kLoopSpeculate = 2;
 
+// Occupancy hint for the register allocator (empty by default).
+#ifndef CWBVH_TRACE_ATTR
+#define CWBVH_TRACE_ATTR
+#endif
+
 template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP>
-__global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
+__global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhParams P) {
     extern __shared__ unsigned long long lds_stack_raw[];
     CWD_LDS unsigned long long* lds_stack = lds_ptr(lds_stack_raw);
     unsigned long long spill[kStackCapacity];
@@ -457,15 +451,6 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
                 }
                 prof(run_tri && tg.y != 0u, pr_tr, pr_tru);
                 if (run_tri && tg.y != 0u) finished = test_triangle();
-                if (run_tri && P.tri_repeat_pct > 0) {
-                    while (true) {
-                        const bool has_t = !finished && tg.y != 0u;
-                        const int holders = __popcll(__ballot(has_t));
-                        if (holders * 100 < P.tri_repeat_pct * __popcll(__ballot(1))) break;
-                        prof(has_t, pr_tr, pr_tru);
-                        if (has_t) finished = test_triangle();
-                    }
-                }
                 // out of inner children: take the next group off the stack
                 if (!finished && ng.y <= 0x00FFFFFFu) {
                     if (tg.y == 0u) {
@@ -516,14 +501,15 @@ __global__ __launch_bounds__(kMaxBlock) void cwbvh_trace(CwbvhParams P) {
         atomicAdd(&pr[4], pr_tr);
         atomicAdd(&pr[5], pr_tru);
     }
-    // last wave out resets the fetch counters for the next launch on this stream
-    // (one-ray-per-thread launches claim nothing, so they skip the exit count)
+    // The last block out resets the fetch counters for the next launch on this
+    // stream (one-ray-per-thread launches claim nothing and skip this).  No
+    // fence is needed: every wave has consumed the result of its last claim
+    // before the barrier, so all claims are performed before the exit count.
     if (!MULTI && !P.persistent) return;
-    __threadfence();
-    const unsigned long long alive = __ballot(1);
-    if (lane == __ffsll((long long)alive) - 1) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
         const unsigned prev = atomicAdd(&P.work[kWorkExited], 1u);
-        if (prev == (unsigned)P.total_waves - 1u) {
+        if (prev == (unsigned)P.total_blocks - 1u) {
             for (int k = 0; k < kWorkSlices; ++k) atomicExch(&P.work[k * kWorkStride], 0u);
             atomicExch(&P.work[kWorkExited], 0u);
         }
@@ -690,10 +676,11 @@ int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_
         grid = (long long)resident_blocks(kernel, block, lds) * cw::device_compute_units();
     }
     p.total_waves = (int)(grid * (block / 64));
+    p.total_blocks = (int)grid;
     if (p.chunk <= 0) {
-        // automatic: about 32 claims per wave over the launch, 64..512 rays each
+        // automatic: about 32 claims per wave over the launch, 128..512 rays each
         long long c = (long long)p.n / ((long long)p.total_waves * 32);
-        c = std::min<long long>(512, std::max<long long>(64, (c + 63) / 64 * 64));
+        c = std::min<long long>(512, std::max<long long>(128, (c + 63) / 64 * 64));
         p.chunk = (int)c;
     }
     hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(block), lds, stream, p);
@@ -754,20 +741,15 @@ int32_t validate_config(int32_t kind, const cwbvh_trace_config& c) {
     }
     if (kind == CWBVH_KIND_CWBVH) {
         if (c.fetch_chunk < 0 || c.fetch_chunk > (1 << 20) || c.grid_blocks < 0 || c.refill_budget < 1 ||
-            c.persistent < 0 || c.persistent > 1 ||
+            c.persistent < 0 || c.persistent > 2 ||
             c.refill_idle < 0 || c.refill_idle > 64 || c.postpone_pct < 0 || c.postpone_pct > 100 ||
             c.prefetch < 0 || c.prefetch > 1 || c.work_slices < 1 || c.work_slices > kWorkSlices ||
-            c.loop_mode < kLoopGroup || c.loop_mode > kLoopSpeculate ||
-            c.tri_repeat_pct < 0 || c.tri_repeat_pct > 100) {
+            c.loop_mode < kLoopGroup || c.loop_mode > kLoopSpeculate) {
             set_error("invalid CWBVH trace configuration");
             return CWBVH_ERR_INVALID_ARGUMENT;
         }
         if (c.postpone_pct > 0 && c.loop_mode == kLoopInterleave) {
             set_error("postpone_pct applies to loop_mode 0 and 2");
-            return CWBVH_ERR_INVALID_ARGUMENT;
-        }
-        if (c.tri_repeat_pct > 0 && c.loop_mode == kLoopGroup) {
-            set_error("tri_repeat_pct applies to loop_mode 1 and 2");
             return CWBVH_ERR_INVALID_ARGUMENT;
         }
     }
@@ -796,9 +778,11 @@ int32_t launch_cwbvh(const TraceLaunch& L) {
     p.postpone_pct = L.cfg.postpone_pct;
     p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
     p.slices = L.cfg.work_slices;
-    p.persistent = L.cfg.persistent;
+    // 2 = automatic: below kAutoPersistentRays the persistent grid's fixed
+    // start-up and exit cost outweighs its load balancing (MI355X, C0 rays:
+    // 262k rays trace 2x faster one per thread, 1M rays 5% slower)
+    p.persistent = L.cfg.persistent == 2 ? (L.n >= cw::kAutoPersistentRays ? 1 : 0) : L.cfg.persistent;
     p.spec_limit = cw::speculation_limit(s->stack_need);
-    p.tri_repeat_pct = L.cfg.tri_repeat_pct;
     return dispatch_cwbvh<false>(p, L.flags, L.cfg, L.stream);
 }
 
@@ -821,7 +805,6 @@ int32_t launch_cwbvh_batch(const BatchLaunch& L) {
     p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
     p.slices = 1;
     p.persistent = 1;
-    p.tri_repeat_pct = L.cfg.tri_repeat_pct;
     return dispatch_cwbvh<true>(p, L.flags, L.cfg, L.stream);
 }
 

@@ -19,7 +19,10 @@ import numpy as np
 
 def main() -> int:
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c3"])
+    ap.add_argument("--c3-tris", type=int, default=5_000_000)
+    ap.add_argument("--c3-rays", type=int, default=8 * 1024 * 1024)
+    ap.add_argument("--loop-mode", type=int, default=None, help="CWBVH schedule (default: library default)")
     ap.add_argument("--launches", type=int, default=2)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--any", action="store_true")
@@ -28,7 +31,15 @@ def main() -> int:
     import cwbvh_amd as cw
     from cwbvh_amd import workloads as W
 
-    wl = W.c0() if args.workload == "c0" else W.c1(W.gpu_primary_tracer())
+    if args.workload == "c0":
+        wl = W.c0()
+    elif args.workload == "c3":
+        wl = W.c3(args.c3_tris, args.c3_rays)
+    else:
+        wl = W.c1(W.gpu_primary_tracer())
+    cfg = None
+    if args.loop_mode is not None:
+        cfg = cw.default_trace_config(cw.KIND_CWBVH, loop_mode=args.loop_mode)
     bvh2, cwb = cw.build_layouts(wl.mesh)
     n = len(wl.rays)
     s_cw, s_b2 = cw.Scene(cwb), cw.Scene(bvh2)
@@ -42,12 +53,12 @@ def main() -> int:
     cw.check(cw.lib.cwbvh_copy_u4(dst.ptr, src.ptr, copy_bytes // 16, None))
     flags = cw.TRACE_ANY if args.any else cw.TRACE_CLOSEST
     for _ in range(args.launches):
-        s_cw.trace(dr.ptr, dh.ptr, n, flags)
+        s_cw.trace(dr.ptr, dh.ptr, n, flags, cfg)
     for _ in range(args.launches):
         s_b2.trace(dr.ptr, dh.ptr, n, flags)
     cw.synchronize()
     print(json.dumps({"workload": wl.name, "rays": n, "copy_bytes": copy_bytes, "launches": args.launches,
-                      "flags": flags, "cwbvh_nodes": cwb.stats()["num_nodes"],
+                      "flags": flags, "cwbvh_nodes": cwb.stats()["num_nodes"], "loop_mode": args.loop_mode,
                       "scene_bytes": {"cwbvh": s_cw.device_bytes(), "bvh2": s_b2.device_bytes()}}))
     return 0
 

@@ -266,16 +266,20 @@ typedef struct cwbvh_trace_config {
                                 this % of the active lanes have triangles; not for loop_mode 1 */
     int32_t prefetch;        /* 0 or 1: issue the next node's loads before the triangle loop */
     int32_t work_slices;     /* 1..8 contiguous ray ranges, one drained first per XCD (CWBVH) */
-    int32_t persistent;      /* CWBVH: 1 = persistent grid with work fetching, 0 = one ray per thread */
+    int32_t persistent;      /* CWBVH: 1 = persistent grid with work fetching, 0 = one ray per thread,
+                                2 = persistent from 2^19 rays up (default) */
     int32_t loop_mode;       /* CWBVH inner-loop schedule: 0 = node visit + its whole triangle group
                                 per iteration (reference order), 1 = node visit or one triangle test
                                 per iteration, 2 = as 1, and lanes holding triangles also visit a
                                 node, parking its triangle group on the stack */
-    int32_t tri_repeat_pct;  /* loop_mode 1 and 2: repeat the triangle test within an iteration while
-                                at least this % of the active lanes hold triangles (0 = one test) */
 } cwbvh_trace_config;
 
 int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
+
+/* The configuration cwbvh_trace uses when cfg is NULL: the kind's default,
+ * with the grouped schedule (loop_mode 0) for CWBVH scenes larger than the
+ * 256 MB Infinity Cache, whose traversal is bound by HBM traffic. */
+int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_config* cfg);
 
 /* Traverse n rays.  d_counters (nullable, CWBVH_TRACE_STATS only) receives
  * uint64[2] = {visited nodes, tested triangles}, accumulated. */

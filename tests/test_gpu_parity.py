@@ -94,10 +94,8 @@ CWBVH_VARIANTS = [
     dict(loop_mode=2, postpone_pct=50),
     dict(loop_mode=2, postpone_pct=90, lds_depth=4, block_size=256),
     dict(loop_mode=2, postpone_pct=100, block_size=64, fetch_chunk=3, prefetch=1),
-    dict(loop_mode=1, tri_repeat_pct=50),
-    dict(loop_mode=1, tri_repeat_pct=1, lds_depth=2, persistent=0),
-    dict(loop_mode=2, tri_repeat_pct=70, postpone_pct=40),
-    dict(loop_mode=2, tri_repeat_pct=100, block_size=256, refill_idle=0, refill_budget=1),
+    dict(loop_mode=1, lds_depth=2, persistent=0),
+    dict(loop_mode=2, postpone_pct=40, block_size=256, refill_idle=0, refill_budget=1),
 ]
 
 
@@ -194,9 +192,8 @@ def test_stats_counters_match_oracle(cw, orc, c0):
             assert np.all(gc >= rc) and np.all(gc <= 1.2 * rc), (k, rc, gc)
 
 
-@pytest.mark.parametrize("loop_mode,postpone,repeat", [(0, 0, 0), (1, 0, 0), (1, 0, 50), (2, 0, 0), (2, 60, 0),
-                                                        (2, 60, 60)])
-def test_stats_counters_loop_modes(cw, orc, c0, loop_mode, postpone, repeat):
+@pytest.mark.parametrize("loop_mode,postpone", [(0, 0), (1, 0), (2, 0), (2, 60)])
+def test_stats_counters_loop_modes(cw, orc, c0, loop_mode, postpone):
     """The grouped and interleaved schedules keep each ray's order of node
     visits and triangle tests, so their counters equal the oracle's.  Speculation tests a
     node's triangles later than the oracle does, so the closest hit shrinks
@@ -206,7 +203,7 @@ def test_stats_counters_loop_modes(cw, orc, c0, loop_mode, postpone, repeat):
     lay = lays["cwbvh"]
     say(f"counters loop_mode={loop_mode}")
     _, rc = oracle_trace(orc, lay, rays, counters=True)
-    cfg = cw.default_trace_config(cw.KIND_CWBVH, loop_mode=loop_mode, postpone_pct=postpone, tri_repeat_pct=repeat)
+    cfg = cw.default_trace_config(cw.KIND_CWBVH, loop_mode=loop_mode, postpone_pct=postpone)
     _, gc = cw.Scene(lay).trace_numpy(rays, 0, cfg, with_stats=True)
     if loop_mode in (0, 1):
         assert np.array_equal(rc, gc), (rc, gc)
@@ -214,8 +211,7 @@ def test_stats_counters_loop_modes(cw, orc, c0, loop_mode, postpone, repeat):
         assert np.all(gc >= 0.9 * rc) and np.all(gc <= 1.5 * rc), (rc, gc)
 
 
-@pytest.mark.parametrize("kw", [dict(loop_mode=0), dict(loop_mode=1), dict(loop_mode=1, tri_repeat_pct=50),
-                                dict(loop_mode=2, postpone_pct=60)])
+@pytest.mark.parametrize("kw", [dict(loop_mode=0), dict(loop_mode=1), dict(loop_mode=2, postpone_pct=60)])
 def test_loop_profile_counters(cw, c0, kw):
     """The occupancy counters of a STATS launch agree with its work counters:
     one node step per visited node, one triangle use per test, and no part of
@@ -234,11 +230,43 @@ def test_loop_profile_counters(cw, c0, kw):
     assert cw.loop_profile()["iteration_slots"] == 0  # cleared on read
 
 
+@pytest.mark.parametrize("count", [1, 63, 100_000, 600_000])
+def test_automatic_persistence_by_ray_count(cw, c0, c0_ref, count):
+    """The default (persistent = 2) traces small batches one ray per thread
+    and large ones with the persistent grid; both match the oracle."""
+    wl, lays = c0
+    say(f"default config on {count} rays")
+    sc = cw.Scene(lays["cwbvh"])
+    assert sc.default_config().persistent == 2
+    for any_hit in (False, True):
+        g = sc.trace_numpy(wl.rays[:count], int(any_hit))
+        assert_bitwise(g, c0_ref[("cwbvh", any_hit)][:count], f"auto persistence n={count} any={any_hit}")
+
+
+def test_wrapped_scene_and_scene_default_config(cw, c0, c0_ref):
+    """A scene wrapped around caller buffers traces like the owning upload,
+    and the per-scene default picks the schedule by scene size."""
+    wl, lays = c0
+    for k in ("cwbvh", "bvh2"):
+        lay = lays[k]
+        say(f"wrapped scene {k}")
+        bufs = [cw.DeviceBuffer.from_numpy(a) for a in (lay.nodes, lay.woop, lay.prim_idx)]
+        sc = cw.Scene.wrap(lay.kind, lay.node_stride, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr,
+                           lay.stats()["stack_need"], keep=bufs)
+        for any_hit in (False, True):
+            assert_bitwise(sc.trace_numpy(wl.rays, int(any_hit)), c0_ref[(k, any_hit)], f"wrapped {k} any={any_hit}")
+    small = cw.Scene(lays["cwbvh"]).default_config()
+    assert small.loop_mode == cw.default_trace_config(cw.KIND_CWBVH).loop_mode == 1
+    # a CWBVH tree larger than the 256 MB Infinity Cache: the grouped schedule
+    big_nodes = cw.DeviceBuffer(300 << 20)
+    big = cw.Scene.wrap(cw.KIND_CWBVH, 5, big_nodes.ptr, big_nodes.ptr, None, 8, keep=(big_nodes,))
+    assert big.default_config().loop_mode == 0
+
+
 def test_invalid_loop_configs_rejected(cw, c0):
     wl, lays = c0
     sc = cw.Scene(lays["cwbvh"])
-    for kw in (dict(loop_mode=3), dict(loop_mode=-1), dict(loop_mode=1, postpone_pct=20),
-               dict(loop_mode=0, tri_repeat_pct=50), dict(loop_mode=1, tri_repeat_pct=101)):
+    for kw in (dict(loop_mode=3), dict(loop_mode=-1), dict(loop_mode=1, postpone_pct=20)):
         cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw)
         with pytest.raises(cw.CwbvhError) as e:
             sc.trace_numpy(wl.rays[:64], 0, cfg)

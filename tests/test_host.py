@@ -52,7 +52,7 @@ def test_argument_validation(cw):
 def test_trace_config_defaults(cw):
     c = cw.default_trace_config(cw.KIND_CWBVH)
     assert c.block_size in (64, 128, 256) and 1 <= c.lds_depth <= cw.STACK_CAPACITY
-    assert c.fetch_chunk >= 0 and c.refill_budget >= 1 and c.persistent in (0, 1)
+    assert c.fetch_chunk >= 0 and c.refill_budget >= 1 and c.persistent in (0, 1, 2)
     assert c.loop_mode in (0, 1, 2)
     b = cw.default_trace_config(cw.KIND_BVH2)
     assert b.block_size in (64, 128, 256)
