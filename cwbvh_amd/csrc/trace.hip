@@ -119,8 +119,9 @@ __device__ __forceinline__ unsigned child_hits4(unsigned meta4, unsigned lox, un
 //    visits a node in the same iteration; the new node's triangle group is
 //    pushed on the stack and tested later, so node visits keep every lane
 //    with inner children busy.  With postpone_pct > 0 the wave also holds
-//    back its triangle step until that share of its lanes have triangles (or
-//    no lane can visit a node), so the step runs with more lanes on.
+//    back its triangle step until that share of its lanes hold triangles and
+//    have no node to visit (or no lane can visit a node), so the step runs
+//    with more lanes on.
 //    A lane speculates only while its stack holds fewer than spec_limit
 //    entries (capacity minus the tree's stack need), so the parked groups
 //    never push the stack past its capacity.
@@ -445,9 +446,11 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                 // once enough lanes hold triangles or a lane holds nothing else)
                 bool run_tri = true;
                 if (LOOP == kLoopSpeculate && P.postpone_pct > 0) {
+                    // lanes that hold triangles and cannot visit a node next
+                    // iteration idle until the triangle step runs
                     const bool can_visit = ng.y > 0x00FFFFFFu && (tg.y == 0u || sp < spec_limit);
-                    const int holders = __popcll(__ballot(tg.y != 0u));
-                    run_tri = !__any(can_visit) || holders * 100 >= P.postpone_pct * __popcll(__ballot(1));
+                    const int stuck = __popcll(__ballot(tg.y != 0u && !can_visit));
+                    run_tri = !__any(can_visit) || stuck * 100 >= P.postpone_pct * __popcll(__ballot(1));
                 }
                 prof(run_tri && tg.y != 0u, pr_tr, pr_tru);
                 if (run_tri && tg.y != 0u) finished = test_triangle();

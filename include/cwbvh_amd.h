@@ -263,7 +263,8 @@ typedef struct cwbvh_trace_config {
     int32_t refill_budget;   /* ... for this many accumulated lane-iterations */
     int32_t postpone_pct;    /* 0 = off; loop_mode 0: postpone a triangle group when fewer than this %
                                 of the lanes are active; loop_mode 2: hold triangle tests back until
-                                this % of the active lanes have triangles; not for loop_mode 1 */
+                                this % of the active lanes hold triangles and no node to visit;
+                                not for loop_mode 1 */
     int32_t prefetch;        /* 0 or 1: issue the next node's loads before the triangle loop */
     int32_t work_slices;     /* 1..8 contiguous ray ranges, one drained first per XCD (CWBVH) */
     int32_t persistent;      /* CWBVH: 1 = persistent grid with work fetching, 0 = one ray per thread,
