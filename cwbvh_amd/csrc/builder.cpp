@@ -75,6 +75,17 @@ struct Builder {
         int32_t n = 0;
     };
 
+    // Bin of centroid coordinate x over [lo, hi] split into nb bins, or -1 when
+    // the extent is not a positive finite number (computed in double, so tiny
+    // or huge extents neither overflow nor produce NaN).
+    static int bin_of(float x, float lo, float hi, int nb) {
+        const double ext = (double)hi - (double)lo;
+        if (!(ext > 0.0) || !std::isfinite(ext)) return -1;
+        const double f = ((double)x - (double)lo) / ext * nb;
+        if (!(f > 0.0)) return 0;
+        return f >= nb - 1 ? nb - 1 : (int)f;
+    }
+
     // Binned SAH over [b, e).  Returns the partition point or -1 for "no good
     // split" (then the caller splits at the median).
     int32_t split(int32_t b, int32_t e, const Box& cb, int depth) {
@@ -83,7 +94,7 @@ struct Builder {
         while ((1 << log2n) < n) ++log2n;
         bool force_median = depth + log2n >= max_depth;
         int best_axis = -1, best_bin = -1;
-        float best_cost = INFINITY;
+        double best_cost = INFINITY;
         if (!force_median) {
             int nb = std::min(bins, std::max(4, n));
             std::vector<Bin> bin(3 * nb);
@@ -91,10 +102,8 @@ struct Builder {
                 for (int32_t i = lo; i < hi; ++i) {
                     int32_t p = refs[i];
                     for (int a = 0; a < 3; ++a) {
-                        float lo_a = comp(cb.lo, a), ext = comp(cb.hi, a) - lo_a;
-                        if (!(ext > 0)) continue;
-                        int k = (int)((comp(pcen[p], a) - lo_a) * ((float)nb / ext));
-                        k = std::min(nb - 1, std::max(0, k));
+                        const int k = bin_of(comp(pcen[p], a), comp(cb.lo, a), comp(cb.hi, a), nb);
+                        if (k < 0) continue;
                         Bin& bb = out[a * nb + k];
                         bb.box.grow(pbox[p]);
                         bb.n++;
@@ -120,7 +129,7 @@ struct Builder {
             } else {
                 bin_range(b, e, bin);
             }
-            std::vector<float> right_cost(nb);
+            std::vector<double> right_cost(nb);
             for (int a = 0; a < 3; ++a) {
                 if (!(comp(cb.hi, a) - comp(cb.lo, a) > 0)) continue;
                 Box acc;
@@ -128,7 +137,7 @@ struct Builder {
                 for (int k = nb - 1; k >= 1; --k) {
                     acc.grow(bin[a * nb + k].box);
                     cnt += bin[a * nb + k].n;
-                    right_cost[k] = cnt ? acc.area() * (float)cnt : 0.f;
+                    right_cost[k] = cnt ? acc.area() * cnt : 0.0;
                 }
                 acc = Box();
                 cnt = 0;
@@ -136,7 +145,7 @@ struct Builder {
                     acc.grow(bin[a * nb + k].box);
                     cnt += bin[a * nb + k].n;
                     if (cnt == 0 || cnt == n) continue;
-                    float c = acc.area() * (float)cnt + right_cost[k + 1];
+                    const double c = acc.area() * cnt + right_cost[k + 1];
                     if (c < best_cost) {
                         best_cost = c;
                         best_axis = a;
@@ -147,11 +156,9 @@ struct Builder {
         }
         if (best_axis < 0) return -1;
         int nb = std::min(bins, std::max(4, n));
-        float lo_a = comp(cb.lo, best_axis), ext = comp(cb.hi, best_axis) - lo_a;
+        const float lo_a = comp(cb.lo, best_axis), hi_a = comp(cb.hi, best_axis);
         auto mid = std::partition(refs.begin() + b, refs.begin() + e, [&](int32_t p) {
-            int k = (int)((comp(pcen[p], best_axis) - lo_a) * ((float)nb / ext));
-            k = std::min(nb - 1, std::max(0, k));
-            return k <= best_bin;
+            return bin_of(comp(pcen[p], best_axis), lo_a, hi_a, nb) <= best_bin;
         });
         int32_t m = (int32_t)(mid - refs.begin());
         if (m == b || m == e) return -1;
@@ -410,9 +417,13 @@ struct Collapse {
         C.assign((size_t)nn * 8, 0.f);
         dec.assign((size_t)nn * 8, 0);
         split8.assign(nn, 0);
+        // areas relative to the root keep the float cost table finite for any
+        // coordinate range
+        const double root_area = nn ? N[T.root].box.area() : 0.0;
+        const double inv_root = root_area > 0.0 && std::isfinite(root_area) ? 1.0 / root_area : 0.0;
         for (int32_t n = nn - 1; n >= 0; --n) {
             const BNode& nd = N[n];
-            float A = nd.box.area();
+            const float A = inv_root > 0.0 ? (float)(nd.box.area() * inv_root) : 0.f;
             float* c = &C[(size_t)n * 8];
             uint8_t* d = &dec[(size_t)n * 8];
             if (nd.left < 0) {
@@ -440,7 +451,9 @@ struct Collapse {
             }
             float inner = A * P.cwbvh_node_cost + dist[8];
             float leafc = nd.prims <= 3 ? A * nd.prims * P.cwbvh_tri_cost : INFINITY;
-            if (leafc <= inner) {
+            // a leaf slot holds at most 3 triangles (unary count in the meta
+            // byte), whatever the costs say
+            if (nd.prims <= 3 && leafc <= inner) {
                 c[1] = leafc;
                 d[1] = 1;
             } else {

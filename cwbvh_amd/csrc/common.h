@@ -54,10 +54,12 @@ struct Box {
     void grow(V3 p) { lo = vmin(lo, p); hi = vmax(hi, p); }
     void grow(const Box& b) { lo = vmin(lo, b.lo); hi = vmax(hi, b.hi); }
     bool empty() const { return !(lo.x <= hi.x); }
-    float area() const {
-        if (empty()) return 0.f;
-        V3 d = hi - lo;
-        return 2.f * (d.x * d.y + d.y * d.z + d.z * d.x);
+    // in double: the surface area of a box with float corners can exceed the
+    // float range (extents above ~1e19), and SAH costs must stay finite
+    double area() const {
+        if (empty()) return 0.0;
+        const double dx = (double)hi.x - lo.x, dy = (double)hi.y - lo.y, dz = (double)hi.z - lo.z;
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
     }
     V3 center() const { return (lo + hi) * 0.5f; }
 };

@@ -194,3 +194,49 @@ def test_sah_quality_cwbvh_fewer_bytes(cw, orc):
     _, c8 = orc.trace_cwbvh(cwb.nodes, 5, cwb.woop, wl.rays, counters=True)
     _, c2 = orc.trace_bvh2(bvh2.nodes, bvh2.woop, wl.rays, counters=True)
     assert 80 * c8[0] + 48 * c8[1] < 0.8 * (64 * c2[0] + 48 * c2[1])
+
+
+def _strip_mesh(cw, xs, y_extent, z_extent, base):
+    """Triangles side by side along x: triangle i spans [xs[i], xs[i+1]] in x,
+    the full y_extent and z_extent, offset by base."""
+    v, idx = [], []
+    for i in range(len(xs) - 1):
+        x0, x1 = xs[i], xs[i + 1]
+        k = len(v)
+        v += [(x0, 0.0, 0.0), (x1, y_extent, 0.0), (x0, y_extent, z_extent)]
+        idx.append((k, k + 1, k + 2))
+    v = np.asarray(v, np.float64) + np.asarray(base, np.float64)
+    return cw.Mesh.from_arrays(v.astype(np.float32), np.asarray(idx, np.int32))
+
+
+@pytest.mark.parametrize("extent,y_extent,z_extent,base", [
+    (255 * 2.0 ** -3, 1.0, 0.0, (1.0, 1.0, 1.0)),                          # exactly 255 * 2^e
+    (255 * 2.0 ** -3 * (1 + 2.0 ** -20), 0.3, 0.25, (0.3, -0.7, 0.1)),     # just above a power of two
+    (255 * 2.0 ** 4 * (1 - 2.0 ** -22), 255 * 2.0 ** 4, 1e-3, (-3.0, 2.0, 5.0)),  # just below
+    (1e-30, 1e-30, 0.0, (0.0, 0.0, 0.0)),                                  # tiny extents
+    (1e30, 3e29, 2e29, (-5e29, -1e29, 0.0)),                               # huge extents
+    (1e-3, 1e-3, 1e-3, (-1e6, 3e5, -2e4)),                                 # small box far from origin
+    (0.0, 0.0, 0.0, (1.0, 2.0, 3.0)),                                      # every triangle one point
+])
+def test_quantization_edge_cases(cw, orc, extent, y_extent, z_extent, base):
+    """Child boxes decoded from the quantized planes contain their triangles
+    at the exponent boundaries the reference converter gets wrong
+    (src/GPUBVHConverter.cpp:284-316: log2f rounding, zero extent)."""
+    xs = np.linspace(0.0, extent, 9)
+    mesh = _strip_mesh(cw, xs, y_extent, z_extent, base)
+    bvh2, cwb = cw.build_layouts(mesh)
+    check_cwbvh(mesh, cwb, orc)
+    check_bvh2(mesh, bvh2)
+    # traversal through the quantized boxes finds what brute force finds
+    lo, hi = mesh.bounds()
+    c = (lo.astype(np.float64) + hi) / 2
+    rng = np.random.default_rng(9)
+    o = c + (hi - lo + 1e-30) * rng.uniform(-0.6, 0.6, size=(256, 3))
+    o[:, 2] = lo[2] - max(extent, 1e-20)
+    d = np.tile([0.0, 0.0, 1.0], (256, 1)) + rng.normal(scale=1e-3, size=(256, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((256, 8), np.float32)
+    rays[:, :3], rays[:, 4:7], rays[:, 7] = o, d, np.inf
+    ref = orc.trace_brute(cwb.woop, np.arange(0, len(cwb.woop), 3, dtype=np.int32)[cwb.prim_idx[::3] >= 0], rays)
+    got = orc.trace_cwbvh(cwb.nodes, cwb.node_stride, cwb.woop, rays)
+    assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
