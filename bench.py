@@ -361,7 +361,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3", "c4"])
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3", "c4", "files"])
+    ap.add_argument("--mesh-dat", help="--config files: the reference's mesh file (src/main.cpp:31-45)")
+    ap.add_argument("--samples-dat", help="--config files: the reference's surfel file (src/main.cpp:52-64)")
+    ap.add_argument("--rays-per-texel", type=int, default=32)
     ap.add_argument("--c3-tris", type=int, default=20_000_000)
     ap.add_argument("--c3-rays", type=int, default=64 * 1024 * 1024)
     ap.add_argument("--c4-scenes", type=int, default=32)
@@ -401,6 +404,11 @@ def main() -> int:
     t0 = time.perf_counter()
     if args.config == "c0":
         wl = W.c0()
+    elif args.config == "files":
+        if not (args.mesh_dat and args.samples_dat):
+            print("--config files needs --mesh-dat and --samples-dat", file=sys.stderr)
+            return 2
+        wl = W.reference_files(args.mesh_dat, args.samples_dat, args.rays_per_texel)
     else:
         wl = W.c1(W.gpu_primary_tracer(params))
     gen_s = time.perf_counter() - t0
@@ -493,7 +501,7 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic",
+        "data": "reference files" if args.config == "files" else "synthetic",
         "config": {"workload": f"{wl.name}: {wl.description}", "triangles": wl.mesh.num_triangles,
                    "rays_per_step": 2 * n, "ao_rays": n, "seed": wl.seed,
                    "step": "CWBVH closest-hit pass + CWBVH any-hit pass over the AO batch",

@@ -15,6 +15,7 @@ Mirrors the reference's host surface:
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 import numpy as np
@@ -171,6 +172,16 @@ _SIGS = {
     "cwbvh_status_string": (C.c_char_p, [_I32]),
     "cwbvh_last_error": (C.c_char_p, []),
     "cwbvh_mesh_from_arrays": (_I32, [_P, _I64, _P, _I64, _PP]),
+    "cwbvh_mesh_read_dat": (_I32, [C.c_char_p, _PP]),
+    "cwbvh_mesh_write_dat": (_I32, [_P, _P, C.c_char_p]),
+    "cwbvh_mesh_materials": (_I32, [_P, _PP, C.POINTER(_I64)]),
+    "cwbvh_samples_read": (_I32, [C.c_char_p, _PP]),
+    "cwbvh_samples_write": (_I32, [C.c_char_p, _I32, _I32, _P, _P, _P]),
+    "cwbvh_samples_view": (_I32, [_P, C.POINTER(_I32), C.POINTER(_I32), _PP, _PP, _PP]),
+    "cwbvh_samples_free": (None, [_P]),
+    "cwbvh_rays_ao_from_samples": (_I32, [_P, _P, _P, _I64, _I32, _F, _F, _U64, _P]),
+    "cwbvh_layout_save": (_I32, [_P, C.c_char_p]),
+    "cwbvh_layout_load": (_I32, [C.c_char_p, _PP]),
     "cwbvh_mesh_view": (_I32, [_P, _PP, C.POINTER(_I64), _PP, C.POINTER(_I64)]),
     "cwbvh_mesh_bounds": (_I32, [_P, _P, _P]),
     "cwbvh_mesh_free": (None, [_P]),
@@ -310,6 +321,25 @@ class Mesh:
         return cls._make(lib.cwbvh_mesh_from_arrays, _ptr(v), len(v), _ptr(i), len(i))
 
     @classmethod
+    def read_dat(cls, path) -> "Mesh":
+        """Read the reference's mesh file format (materials kept, see .materials)."""
+        return cls._make(lib.cwbvh_mesh_read_dat, os.fsencode(path))
+
+    def write_dat(self, path, materials=None) -> None:
+        m = None if materials is None else np.ascontiguousarray(materials, np.int32).reshape(-1)
+        if m is not None and len(m) != self.num_triangles:
+            raise ValueError("one material per triangle")
+        check(lib.cwbvh_mesh_write_dat(self._h, _ptr(m) if m is not None else None, os.fsencode(path)))
+
+    @property
+    def materials(self) -> np.ndarray:
+        p, n = C.c_void_p(), C.c_int64()
+        check(lib.cwbvh_mesh_materials(self._h, C.byref(p), C.byref(n)))
+        if n.value == 0:
+            return np.zeros(0, np.int32)
+        return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_int32)), shape=(n.value,)).copy()
+
+    @classmethod
     def heightfield(cls, nx=256, ny=128, octaves=8, amp0=0.1, seed=0) -> "Mesh":
         return cls._make(lib.cwbvh_mesh_heightfield, nx, ny, octaves, amp0, seed)
 
@@ -376,6 +406,46 @@ def rays_ao_surface(mesh: Mesh, num_points: int, rays_per_point: int, offset: fl
                     seed: int = 0) -> np.ndarray:
     out = np.empty((num_points * rays_per_point, 8), np.float32)
     check(lib.cwbvh_rays_ao_surface(mesh.handle, num_points, rays_per_point, offset, tmin, tmax, seed, _ptr(out)))
+    return out
+
+
+def read_samples(path):
+    """Read the reference's surfel file: (positions (n,4), normals (n,4), radius (n,), (size_x, size_y))."""
+    h = C.c_void_p()
+    check(lib.cwbvh_samples_read(os.fsencode(path), C.byref(h)))
+    try:
+        sx, sy = C.c_int32(), C.c_int32()
+        pp, pn, pr = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        check(lib.cwbvh_samples_view(h, C.byref(sx), C.byref(sy), C.byref(pp), C.byref(pn), C.byref(pr)))
+        n = sx.value * sy.value
+        if n == 0:
+            return np.zeros((0, 4), np.float32), np.zeros((0, 4), np.float32), np.zeros(0, np.float32), (sx.value, sy.value)
+        f4 = lambda p: np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(n, 4)).copy()
+        rad = np.ctypeslib.as_array(C.cast(pr, C.POINTER(C.c_float)), shape=(n,)).copy()
+        return f4(pp), f4(pn), rad, (sx.value, sy.value)
+    finally:
+        lib.cwbvh_samples_free(h)
+
+
+def write_samples(path, positions, normals, radius, size_x: int, size_y: int) -> None:
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1, 4)
+    n = np.ascontiguousarray(normals, np.float32).reshape(-1, 4)
+    r = np.ascontiguousarray(radius, np.float32).reshape(-1)
+    if not (len(p) == len(n) == len(r) == size_x * size_y):
+        raise ValueError("size_x * size_y samples of each array")
+    check(lib.cwbvh_samples_write(os.fsencode(path), size_x, size_y, _ptr(p), _ptr(n), _ptr(r)))
+
+
+def rays_ao_from_samples(positions, normals, radius, rays_per_texel: int = 32, tmin: float = 0.01,
+                         tmax: float = 1e20, seed: int = 0) -> np.ndarray:
+    """AO rays per surfel with the reference's frame, origin offset and ray
+    interval (src/ValidationKernels.cu:63-101), on the library's random stream."""
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1, 4)
+    n = np.ascontiguousarray(normals, np.float32).reshape(-1, 4)
+    r = np.ascontiguousarray(radius, np.float32).reshape(-1)
+    out = np.empty((len(p) * rays_per_texel, 8), np.float32)
+    check(lib.cwbvh_rays_ao_from_samples(_ptr(p), _ptr(n), _ptr(r), len(p), rays_per_texel, tmin, tmax, seed,
+                                         _ptr(out)))
     return out
 
 
@@ -492,6 +562,16 @@ class Layout:
         return {"kind": v.kind, "node_stride": v.node_stride, "nodes_f4": v.nodes_f4, "woop_f4": v.woop_f4,
                 "max_depth": v.max_depth, "stack_need": v.stack_need, "num_nodes": v.num_nodes,
                 "num_leaves": v.num_leaves, "num_tri_refs": v.num_tri_refs, "sah_cost": v.sah_cost}
+
+    def save(self, path) -> None:
+        """Write the layout cache file (cwbvh_layout_save)."""
+        check(lib.cwbvh_layout_save(self._h, os.fsencode(path)))
+
+    @classmethod
+    def load(cls, path) -> "Layout":
+        out = C.c_void_p()
+        check(lib.cwbvh_layout_load(os.fsencode(path), C.byref(out)))
+        return cls(out.value)
 
     def restride(self, node_stride: int) -> "Layout":
         out = C.c_void_p()

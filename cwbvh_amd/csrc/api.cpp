@@ -81,6 +81,7 @@ const char* cwbvh_status_string(int32_t s) {
         case CWBVH_ERR_BUDGET: return "request exceeds the memory budget";
         case CWBVH_ERR_CANCELLED: return "cancelled";
         case CWBVH_ERR_INTERNAL: return "internal error";
+        case CWBVH_ERR_IO: return "file error";
         default: return "unknown status";
     }
 }

@@ -708,6 +708,88 @@ int32_t cwbvh_build_layouts(const cwbvh_mesh* mesh, const cwbvh_build_params* pa
     return CWBVH_OK;
 }
 
+namespace {
+
+// Structural checks of caller-supplied layouts: every index the kernels derive
+// from a reachable node stays inside the arrays, so a malformed array is
+// refused on the host instead of being read out of bounds on the device.
+// Each node may be reached once (a tree), which also bounds the walk.
+bool validate_cwbvh(cwbvh_layout& L, int32_t& maxd, std::string& why) {
+    const int64_t S = L.node_stride, nn = L.num_nodes, ntri = (int64_t)L.woop.size() / 3;
+    std::vector<uint8_t> seen((size_t)nn, 0);
+    std::vector<std::pair<int64_t, int32_t>> st{{0, 1}};
+    while (!st.empty()) {
+        auto [n, d] = st.back();
+        st.pop_back();
+        if (n < 0 || n >= nn) { why = "child index out of range"; return false; }
+        if (seen[(size_t)n]) { why = "node " + std::to_string(n) + " reached twice"; return false; }
+        if (d > CWBVH_STACK_CAPACITY * 8) { why = "tree too deep"; return false; }
+        seen[(size_t)n] = 1;
+        maxd = std::max(maxd, d);
+        const cwbvh_float4* nd = &L.nodes[(size_t)(n * S)];
+        const uint8_t imask = reinterpret_cast<const uint8_t*>(&nd[0])[15];
+        int32_t base, tbase;
+        std::memcpy(&base, &nd[1].x, 4);
+        std::memcpy(&tbase, &nd[1].y, 4);
+        const uint8_t* meta = reinterpret_cast<const uint8_t*>(&nd[1]) + 8;
+        int k = 0;
+        for (int s = 0; s < 8; ++s) {
+            const unsigned m = meta[s];
+            const bool inner_meta = (m & 0x18u) == 0x18u;
+            if (imask & (1u << s)) {
+                if (m != (0x20u | 0x18u | (unsigned)s)) { why = "inner slot meta does not match imask"; return false; }
+                st.push_back({(int64_t)base + k++, d + 1});
+            } else if (m != 0) {
+                const unsigned unary = m >> 5, off = m & 31u;
+                const int cnt = unary == 1 ? 1 : unary == 3 ? 2 : unary == 7 ? 3 : 0;
+                if (inner_meta || cnt == 0 || off + (unsigned)cnt > 24u) { why = "bad leaf slot meta"; return false; }
+                const int64_t first = (int64_t)tbase + off;
+                if (tbase < 0 || first + cnt > ntri) { why = "leaf triangles out of range"; return false; }
+                L.num_leaves++;
+            }
+        }
+    }
+    return true;
+}
+
+bool validate_bvh2(cwbvh_layout& L, int32_t& maxd, std::string& why) {
+    const int64_t nf4 = (int64_t)L.nodes.size(), wf4 = (int64_t)L.woop.size();
+    std::vector<uint8_t> seen((size_t)(nf4 / 4 + 1), 0);
+    std::vector<std::pair<int64_t, int32_t>> st{{0, 1}};
+    while (!st.empty()) {
+        auto [a, d] = st.back();
+        st.pop_back();
+        if (a < 0 || a % 4 != 0 || a + 3 >= nf4) { why = "node address out of range"; return false; }
+        if (seen[(size_t)(a / 4)]) { why = "node reached twice"; return false; }
+        if (d > CWBVH_BVH2_STACK_CAPACITY * 8) { why = "tree too deep"; return false; }
+        seen[(size_t)(a / 4)] = 1;
+        maxd = std::max(maxd, d);
+        int32_t c[2];
+        std::memcpy(&c[0], &L.nodes[(size_t)a + 3].x, 4);
+        std::memcpy(&c[1], &L.nodes[(size_t)a + 3].y, 4);
+        for (int j = 0; j < 2; ++j) {
+            if (c[j] >= 0) {
+                st.push_back({c[j], d + 1});
+                continue;
+            }
+            // a leaf: Woop rows of 3 float4 closed by a 0x80000000 terminator
+            int64_t w = ~(int64_t)c[j];
+            for (;;) {
+                if (w >= wf4) { why = "leaf runs past the Woop array"; return false; }
+                uint32_t x;
+                std::memcpy(&x, &L.woop[(size_t)w].x, 4);
+                if (x == 0x80000000u) break;
+                if (w + 2 >= wf4) { why = "leaf runs past the Woop array"; return false; }
+                w += 3;
+            }
+            L.num_leaves++;
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
 int32_t cwbvh_layout_from_arrays(int32_t kind, int32_t node_stride, const cwbvh_float4* nodes, int64_t nodes_f4,
                                  const cwbvh_float4* woop, int64_t woop_f4, const int32_t* prim_idx,
                                  int64_t prim_idx_len, cwbvh_layout** out) {
@@ -718,6 +800,7 @@ int32_t cwbvh_layout_from_arrays(int32_t kind, int32_t node_stride, const cwbvh_
     CW_CHECK_ARG(nodes_f4 >= node_stride && nodes_f4 % node_stride == 0 && nodes, "bad node array");
     CW_CHECK_ARG(woop_f4 >= 0 && (woop_f4 == 0 || woop), "bad woop array");
     CW_CHECK_ARG(prim_idx_len >= 0 && (prim_idx_len == 0 || prim_idx), "bad prim index array");
+    CW_CHECK_ARG(prim_idx_len == 0 || prim_idx_len >= woop_f4, "prim index array shorter than the Woop array");
     auto* L = new cwbvh_layout();
     L->kind = kind;
     L->node_stride = node_stride;
@@ -725,40 +808,15 @@ int32_t cwbvh_layout_from_arrays(int32_t kind, int32_t node_stride, const cwbvh_
     L->woop.assign(woop, woop + woop_f4);
     L->prim_idx.assign(prim_idx, prim_idx + prim_idx_len);
     L->num_nodes = nodes_f4 / node_stride;
-    // depth: walk the tree
+    std::string why;
     int32_t maxd = 0;
-    if (kind == CWBVH_KIND_BVH2) {
-        std::vector<std::pair<int32_t, int32_t>> st{{0, 1}};
-        while (!st.empty()) {
-            auto [a, d] = st.back();
-            st.pop_back();
-            if (a < 0 || a + 3 >= nodes_f4 || d > 1024) { delete L; set_error("malformed BVH2 node array"); return CWBVH_ERR_INVALID_ARGUMENT; }
-            maxd = std::max(maxd, d);
-            int32_t c[2];
-            std::memcpy(&c[0], &L->nodes[a + 3].x, 4);
-            std::memcpy(&c[1], &L->nodes[a + 3].y, 4);
-            for (int k = 0; k < 2; ++k)
-                if (c[k] >= 0) st.push_back({c[k], d + 1});
-                else L->num_leaves++;
-        }
-        L->stack_need = maxd + 1;
-    } else {
-        std::vector<std::pair<int64_t, int32_t>> st{{0, 1}};
-        while (!st.empty()) {
-            auto [n, d] = st.back();
-            st.pop_back();
-            if (n < 0 || (n + 1) * node_stride > nodes_f4 || d > 1024) { delete L; set_error("malformed CWBVH node array"); return CWBVH_ERR_INVALID_ARGUMENT; }
-            maxd = std::max(maxd, d);
-            const uint8_t* n0b = reinterpret_cast<const uint8_t*>(&L->nodes[n * node_stride]);
-            int32_t base;
-            std::memcpy(&base, &L->nodes[n * node_stride + 1].x, 4);
-            uint8_t imask = n0b[15];
-            int k = 0;
-            for (int s = 0; s < 8; ++s)
-                if (imask & (1u << s)) st.push_back({(int64_t)base + k++, d + 1});
-        }
-        L->stack_need = maxd + 1;
+    const bool ok = kind == CWBVH_KIND_BVH2 ? validate_bvh2(*L, maxd, why) : validate_cwbvh(*L, maxd, why);
+    if (!ok) {
+        delete L;
+        set_error("malformed " + std::string(kind == CWBVH_KIND_BVH2 ? "BVH2" : "CWBVH") + " layout: " + why);
+        return CWBVH_ERR_INVALID_ARGUMENT;
     }
+    L->stack_need = maxd + 1;
     L->max_depth = maxd;
     for (int32_t v : L->prim_idx)
         if (v >= 0) L->num_tri_refs++;

@@ -103,6 +103,7 @@ struct Rng {
 struct cwbvh_mesh {
     std::vector<float> vertices;   // 3 per vertex
     std::vector<int32_t> indices;  // 3 per triangle
+    std::vector<int32_t> materials;  // one per triangle when read from a mesh file, else empty
     int64_t nv() const { return (int64_t)vertices.size() / 3; }
     int64_t nt() const { return (int64_t)indices.size() / 3; }
     cw::V3 vert(int64_t i) const { return cw::V3{vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]}; }

@@ -23,7 +23,8 @@ from typing import Callable
 import numpy as np
 
 from . import (KIND_CWBVH, Layout, Mesh, Scene, TRACE_CLOSEST, build_layouts, default_build_params, hit_ids,
-               rays_ao_from_hits, rays_ao_surface, rays_pinhole, rays_sphere_from_surface)
+               rays_ao_from_hits, rays_ao_from_samples, rays_ao_surface, rays_pinhole, rays_sphere_from_surface,
+               read_samples)
 
 SEEDS = {"C0": 0xC0C0_0001, "C1": 0xC1C1_0001, "C2": 0xC2C2_0001, "C3": 0xC3C3_0001, "C4": 0xC4C4_0001}
 
@@ -39,6 +40,18 @@ class Workload:
     seed: int
     description: str
     extra: dict = field(default_factory=dict)
+
+
+def reference_files(mesh_path, samples_path, rays_per_texel: int = 32, seed: int = 0) -> Workload:
+    """The reference's own workload (src/main.cpp:31-73): the mesh file and
+    rays_per_texel AO rays per surfel of the sample file (tmin 0.01, tmax
+    1e20), on the library's random stream."""
+    m = Mesh.read_dat(mesh_path)
+    pos, nrm, rad, (sx, sy) = read_samples(samples_path)
+    rays = rays_ao_from_samples(pos, nrm, rad, rays_per_texel, 0.01, 1e20, seed)
+    return Workload("files", m, rays, m.diagonal(), seed,
+                    f"reference files: {m.num_triangles} tris, {sx}x{sy} surfels x {rays_per_texel} AO rays",
+                    extra={"surfels": sx * sy})
 
 
 def c0_mesh(seed: int = SEEDS["C0"]) -> Mesh:

@@ -58,7 +58,8 @@ typedef enum cwbvh_status {
     CWBVH_ERR_NO_DEVICE = 6,
     CWBVH_ERR_BUDGET = 7,          /* request larger than the memory budget */
     CWBVH_ERR_CANCELLED = 8,
-    CWBVH_ERR_INTERNAL = 9
+    CWBVH_ERR_INTERNAL = 9,
+    CWBVH_ERR_IO = 10              /* file cannot be opened, read or written, or its size is wrong */
 } cwbvh_status;
 
 enum {
@@ -115,6 +116,28 @@ int32_t cwbvh_mesh_icosphere_grid(int32_t gx, int32_t gy, int32_t gz, int32_t fr
 int32_t cwbvh_mesh_cluster_soup(int64_t num_triangles, int32_t num_clusters,
                                 float sigma_min, float sigma_max, float edge_mu,
                                 float edge_sigma, uint64_t seed, cwbvh_mesh** out);
+/* The reference's scene file (src/main.cpp:31-45): int32 vertex count,
+ * int32 triangle count, float3 positions, int3 indices, int32 material per
+ * triangle (little-endian, packed).  The reader refuses files whose size does
+ * not match the header (CWBVH_ERR_IO) and out-of-range indices; materials
+ * are kept with the mesh.  The writer stores `materials` (nullable: the
+ * mesh's own, else zeros). */
+int32_t cwbvh_mesh_read_dat(const char* path, cwbvh_mesh** out);
+int32_t cwbvh_mesh_write_dat(const cwbvh_mesh* mesh, const int32_t* materials, const char* path);
+/* Per-triangle materials (count 0 unless the mesh came from a mesh file). */
+int32_t cwbvh_mesh_materials(const cwbvh_mesh* mesh, const int32_t** materials, int64_t* count);
+
+/* The reference's surfel file (src/main.cpp:52-64): int32 size x, int32 size
+ * y, then size x * size y float4 world positions, float4 world normals and
+ * float texel radii. */
+typedef struct cwbvh_samples cwbvh_samples;
+int32_t cwbvh_samples_read(const char* path, cwbvh_samples** out);
+int32_t cwbvh_samples_write(const char* path, int32_t size_x, int32_t size_y, const cwbvh_float4* positions,
+                            const cwbvh_float4* normals, const float* radius);
+int32_t cwbvh_samples_view(const cwbvh_samples* samples, int32_t* size_x, int32_t* size_y,
+                           const cwbvh_float4** positions, const cwbvh_float4** normals, const float** radius);
+void cwbvh_samples_free(cwbvh_samples* samples);
+
 /* Known-answer scene: triangle i lies in plane z = i with vertices
  * (-1.1,-1.1,i), (1.1,-1.1,i), (1.1,1.1,i). */
 int32_t cwbvh_mesh_validation_planes(int32_t count, cwbvh_mesh** out);
@@ -139,6 +162,17 @@ int32_t cwbvh_rays_ao_from_hits(const cwbvh_mesh* mesh, const cwbvh_ray* primary
                                 const cwbvh_hit* hits, const int32_t* prim_ids, int64_t n,
                                 int32_t rays_per_hit, float offset, float tmax,
                                 uint64_t seed, cwbvh_ray* out, int64_t* out_count);
+/* AO rays from surfels, as the reference generates them from its sample file
+ * (src/ValidationKernels.cu:63-101): for texel i and k < rays_per_texel, a
+ * cosine-weighted direction (concentric disk lifted to the hemisphere) in the
+ * frame t1 = normalize(n x z) (n x y when that is shorter than 0.1),
+ * t2 = normalize(t1 x n), and origin p + n * radius * 0.5.  The reference
+ * uses tmin 0.01, tmax 1e20 and curand; the random stream here is the
+ * library's own (seeded), so the same distribution, not the same rays.
+ * out[i * rays_per_texel + k]. */
+int32_t cwbvh_rays_ao_from_samples(const cwbvh_float4* positions, const cwbvh_float4* normals,
+                                   const float* radius, int64_t num_texels, int32_t rays_per_texel,
+                                   float tmin, float tmax, uint64_t seed, cwbvh_ray* out);
 /* Diffuse secondary rays: area-uniform surface points, uniform sphere
  * directions, origin offset along the normal on the side of the direction. */
 int32_t cwbvh_rays_sphere_from_surface(const cwbvh_mesh* mesh, int64_t num_rays, float offset,
@@ -194,6 +228,12 @@ int32_t cwbvh_layout_from_arrays(int32_t kind, int32_t node_stride, const cwbvh_
                                  const int32_t* prim_idx, int64_t prim_idx_len,
                                  cwbvh_layout** out);
 int32_t cwbvh_layout_get_view(const cwbvh_layout* layout, cwbvh_layout_view* view);
+/* Layout cache: write a layout to a file / read it back.  The file holds a
+ * header, the node, Woop and prim-index arrays and a 64-bit FNV-1a checksum;
+ * loading checks size and checksum and validates the tree like
+ * cwbvh_layout_from_arrays, so a scene can be reloaded instead of rebuilt. */
+int32_t cwbvh_layout_save(const cwbvh_layout* layout, const char* path);
+int32_t cwbvh_layout_load(const char* path, cwbvh_layout** out);
 /* Re-lay a CWBVH node array to another node stride (5 <-> 8 float4). */
 int32_t cwbvh_layout_restride(const cwbvh_layout* in, int32_t node_stride, cwbvh_layout** out);
 void cwbvh_layout_free(cwbvh_layout* layout);

@@ -240,3 +240,43 @@ def test_quantization_edge_cases(cw, orc, extent, y_extent, z_extent, base):
     ref = orc.trace_brute(cwb.woop, np.arange(0, len(cwb.woop), 3, dtype=np.int32)[cwb.prim_idx[::3] >= 0], rays)
     got = orc.trace_cwbvh(cwb.nodes, cwb.node_stride, cwb.woop, rays)
     assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+
+
+def test_from_arrays_refuses_malformed_layouts(cw):
+    """Caller arrays are walked on the host before any kernel sees them:
+    indices a kernel would derive out of range are refused, not traced."""
+    mesh = cw.Mesh.heightfield(8, 8, 4, 0.1, seed=5)
+    bvh2, cwb = cw.build_layouts(mesh)
+
+    def refuses(kind, stride, nodes, woop, prim, what):
+        with pytest.raises(cw.CwbvhError) as e:
+            cw.Layout.from_arrays(kind, stride, nodes, woop, prim)
+        assert e.value.status == 1 and what in str(e.value), str(e.value)
+
+    # CWBVH: leaf triangles beyond the Woop array
+    refuses(cw.KIND_CWBVH, 5, cwb.nodes, cwb.woop[:9], cwb.prim_idx[:9], "out of range")
+    # CWBVH: inner meta naming another slot than its imask bit
+    n = cwb.nodes.copy()
+    meta = n[1].view(np.uint8)[8:16]
+    imask = n[0].view(np.uint8)[15]
+    s = next(k for k in range(8) if imask >> k & 1)
+    meta[s] = 0x38 | ((s + 1) % 8)
+    refuses(cw.KIND_CWBVH, 5, n, cwb.woop, cwb.prim_idx, "meta")
+    # CWBVH: a child pointing back at the root (a cycle)
+    n = cwb.nodes.copy()
+    n[1].view(np.int32)[0] = 0
+    refuses(cw.KIND_CWBVH, 5, n, cwb.woop, cwb.prim_idx, "twice")
+    # BVH2: a child address that is not a node boundary
+    n = bvh2.nodes.copy()
+    c = n[3].view(np.int32)
+    j = 0 if c[0] >= 0 else 1
+    c[j] = c[j] + 1 if c[j] >= 0 else 1
+    refuses(cw.KIND_BVH2, 4, n, bvh2.woop, bvh2.prim_idx, "node address")
+    # BVH2: a leaf whose terminator is cut off
+    refuses(cw.KIND_BVH2, 4, bvh2.nodes, bvh2.woop[:5], bvh2.prim_idx[:5], "Woop array")
+    # prim index shorter than the Woop array
+    with pytest.raises(cw.CwbvhError):
+        cw.Layout.from_arrays(cw.KIND_CWBVH, 5, cwb.nodes, cwb.woop, cwb.prim_idx[:-3])
+    # the untouched arrays are accepted
+    assert cw.Layout.from_arrays(cw.KIND_CWBVH, 5, cwb.nodes, cwb.woop, cwb.prim_idx).stats()["num_nodes"] == \
+        cwb.stats()["num_nodes"]
