@@ -276,6 +276,102 @@ def bench_c3(args, cw, W, dev) -> int:
     return 0
 
 
+def bench_ao(args, cw, W, dev) -> int:
+    """AO from surfels on the C1 scene: the primary hits of the 1920x1080
+    camera become surfels (hit point, geometric normal facing the camera,
+    texel radius 2e-4 * diag), 16 AO rays each (tmin 0, tmax 0.05 * diag).
+    Three ways to run the same rays, closest and any hit:
+      unfused      ray generation kernel writes rays, then cwbvh_trace;
+      fused_hits   cwbvh_trace_surfels generates rays in the traversal kernel;
+      fused_counts as fused_hits, with per-surfel hit counts instead of hits."""
+    mesh = W.c1_mesh()
+    diag = mesh.diagonal()
+    prim = W.c1_primary()
+    hits, pids = W.gpu_primary_tracer()(mesh, prim)
+    sel = np.nonzero(pids >= 0)[0]
+    v, idx = mesh.vertices, mesh.indices
+    tri = v[idx[pids[sel]]].astype(np.float64)
+    nrm = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    dirs = prim[sel, 4:7].astype(np.float64)
+    nrm[np.einsum("ij,ij->i", nrm, dirs) > 0] *= -1
+    pos = np.zeros((len(sel), 4), np.float32)
+    pos[:, :3] = prim[sel, :3] + hits[sel, :1] * prim[sel, 4:7]
+    nrm4 = np.zeros((len(sel), 4), np.float32)
+    nrm4[:, :3] = nrm
+    rad = np.full(len(sel), 2e-4 * diag, np.float32)
+    k, tmax, seed = 16, np.float32(0.05 * diag), W.SEEDS["C1"] + 7
+    ntex, n = len(sel), len(sel) * 16
+    _, cwb = cw.build_layouts(mesh, bvh2=False)
+    sc = cw.Scene(cwb)
+    bufs = [cw.DeviceBuffer.from_numpy(a) for a in (pos, nrm4, rad)]
+    d_rays = cw.DeviceBuffer(32 * n)
+    d_hits = cw.DeviceBuffer(16 * n)
+    d_counts = cw.DeviceBuffer(4 * ntex)
+    cfg = sc.default_config()
+    log(f"AO surfels: {ntex} surfels x {k} = {n} rays")
+
+    def unfused(fl):
+        def f():
+            cw.rays_ao_from_samples_device(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, ntex, k, 0.0, tmax, seed, d_rays.ptr)
+            sc.trace(d_rays.ptr, d_hits.ptr, n, fl, cfg)
+        return f
+
+    def fused_hits(fl):
+        return lambda: sc.trace_surfels(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, ntex, k, 0.0, tmax, seed, fl,
+                                        d_hits=d_hits.ptr, cfg=cfg)
+
+    def fused_counts(fl):
+        def f():
+            d_counts.zero()
+            sc.trace_surfels(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, ntex, k, 0.0, tmax, seed, fl,
+                             d_texel_hits=d_counts.ptr, cfg=cfg)
+        return f
+
+    table = {}
+    for name, mk in (("unfused", unfused), ("fused_hits", fused_hits), ("fused_counts", fused_counts)):
+        for fname, fl in (("closest", cw.TRACE_CLOSEST), ("any", cw.TRACE_ANY)):
+            _, per = time_launches([mk(fl)], args.steps, args.warmup, cw)
+            ms = float(per[:, 0].mean())
+            table[f"{name}_{fname}"] = {"ms": round(ms, 4), "mrays_per_s": round(n / ms / 1e3, 1)}
+            log(f"{name} {fname}: {ms:.3f} ms {n / ms / 1e3:.0f} Mrays/s")
+    # the fused kernel returns what tracing the generated rays returns
+    sc.trace_surfels(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, ntex, k, 0.0, tmax, seed, cw.TRACE_CLOSEST,
+                     d_hits=d_hits.ptr, cfg=cfg)
+    cw.synchronize()
+    fused = d_hits.download((n, 4), np.float32)
+    cw.rays_ao_from_samples_device(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, ntex, k, 0.0, tmax, seed, d_rays.ptr)
+    sc.trace(d_rays.ptr, d_hits.ptr, n, cw.TRACE_CLOSEST, cfg)
+    cw.synchronize()
+    identical = bool(np.array_equal(fused.view(np.uint32), d_hits.download((n, 4), np.float32).view(np.uint32)))
+    best = max(table.items(), key=lambda kv: kv[1]["mrays_per_s"])
+    out = {
+        "metric": METRIC + " [AO from surfels, fused ray generation]",
+        "value": table["fused_counts_any"]["mrays_per_s"],
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": table["fused_counts_any"]["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"C1 scene, {ntex} surfels (primary hits) x {k} AO rays, tmax=0.05*diag",
+                   "triangles": mesh.num_triangles, "surfels": ntex, "rays": n, "seed": seed},
+        "results": table,
+        "fused_hits_identical_to_unfused": identical,
+        "speedup_fused_counts_vs_unfused_any": round(table["unfused_any"]["ms"] / table["fused_counts_any"]["ms"], 3),
+        "speedup_fused_hits_vs_unfused_closest": round(table["unfused_closest"]["ms"] / table["fused_hits_closest"]["ms"], 3),
+        "bytes_per_ray": {"unfused": 32 + 32 + 16, "fused_hits": 16, "fused_counts": "4 per surfel"},
+        "best": best[0],
+        "device": dev["arch"],
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_c4(args, cw, W, dev) -> int:
     """C4: resident scenes, Poisson AO query requests (log-uniform 1k..1M rays,
     uniform scene), host-resident rays copied in and hits copied out per
@@ -361,7 +457,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3", "c4", "files"])
+    ap.add_argument("--config", default="c1", choices=["c0", "c1", "c2", "c3", "c4", "files", "ao"])
     ap.add_argument("--mesh-dat", help="--config files: the reference's mesh file (src/main.cpp:31-45)")
     ap.add_argument("--samples-dat", help="--config files: the reference's surfel file (src/main.cpp:52-64)")
     ap.add_argument("--rays-per-texel", type=int, default=32)
@@ -399,6 +495,8 @@ def main() -> int:
         return bench_c4(args, cw, W, dev)
     if args.config == "c3":
         return bench_c3(args, cw, W, dev)
+    if args.config == "ao":
+        return bench_ao(args, cw, W, dev)
 
     params = cw.default_build_params(cwbvh_node_stride=args.node_stride)
     t0 = time.perf_counter()

@@ -181,6 +181,8 @@ _SIGS = {
     "cwbvh_samples_free": (None, [_P]),
     "cwbvh_rays_ao_from_samples": (_I32, [_P, _P, _P, _I64, _I32, _F, _F, _U64, _P]),
     "cwbvh_layout_save": (_I32, [_P, C.c_char_p]),
+    "cwbvh_rays_ao_from_samples_device": (_I32, [_P, _P, _P, _I64, _I32, _F, _F, _U64, _P, _P]),
+    "cwbvh_trace_surfels": (_I32, [_P, _P, _P, _P, _I64, _I32, _F, _F, _U64, _U32, _P, _P, _P, _P, _P]),
     "cwbvh_layout_load": (_I32, [C.c_char_p, _PP]),
     "cwbvh_mesh_view": (_I32, [_P, _PP, C.POINTER(_I64), _PP, C.POINTER(_I64)]),
     "cwbvh_mesh_bounds": (_I32, [_P, _P, _P]),
@@ -425,6 +427,12 @@ def read_samples(path):
         return f4(pp), f4(pn), rad, (sx.value, sy.value)
     finally:
         lib.cwbvh_samples_free(h)
+
+
+def rays_ao_from_samples_device(d_pos: int, d_nrm: int, d_rad: int, num_texels: int, rays_per_texel: int,
+                                tmin: float, tmax: float, seed: int, d_out: int, stream: Stream | None = None) -> None:
+    check(lib.cwbvh_rays_ao_from_samples_device(d_pos, d_nrm, d_rad, num_texels, rays_per_texel, tmin, tmax, seed,
+                                                d_out, stream.handle if stream else None))
 
 
 def write_samples(path, positions, normals, radius, size_x: int, size_y: int) -> None:
@@ -774,6 +782,15 @@ class Scene:
             check(lib.cwbvh_trace(self._s, d_rays, d_hits, n, flags, cfgp, d_counters, st))
         else:
             check(lib.cwbvh_trace_ordered(self._s, d_rays, d_order, d_hits, n, flags, cfgp, d_counters, st))
+
+    def trace_surfels(self, d_pos: int, d_nrm: int, d_rad: int, num_texels: int, rays_per_texel: int, tmin: float,
+                      tmax: float, seed: int, flags: int = TRACE_CLOSEST, d_hits: int | None = None,
+                      d_texel_hits: int | None = None, cfg: TraceConfig | None = None, d_counters: int | None = None,
+                      stream: Stream | None = None) -> None:
+        """Fused AO: rays generated from surfels inside the traversal kernel."""
+        check(lib.cwbvh_trace_surfels(self._s, d_pos, d_nrm, d_rad, num_texels, rays_per_texel, tmin, tmax, seed,
+                                      flags, d_hits, d_texel_hits, C.byref(cfg) if cfg is not None else None,
+                                      d_counters, stream.handle if stream else None))
 
     def trace_numpy(self, rays: np.ndarray, flags: int = TRACE_CLOSEST, cfg: TraceConfig | None = None,
                     order: np.ndarray | None = None, with_stats: bool = False):

@@ -437,6 +437,57 @@ int32_t cwbvh_trace(const cwbvh_scene* scene, const cwbvh_ray* d_rays, cwbvh_hit
     return trace_impl(scene, d_rays, nullptr, d_hits, n, flags, cfg, d_counters, stream);
 }
 
+int32_t cwbvh_rays_ao_from_samples_device(const cwbvh_float4* d_positions, const cwbvh_float4* d_normals,
+                                          const float* d_radius, int64_t num_texels, int32_t rays_per_texel,
+                                          float tmin, float tmax, uint64_t seed, cwbvh_ray* d_out, void* stream) {
+    CW_CHECK_ARG(num_texels >= 0 && rays_per_texel >= 1, "bad ray count");
+    CW_CHECK_ARG(num_texels * (int64_t)rays_per_texel <= 0x7FFFFFFF, "more than 2^31 - 1 rays");
+    CW_CHECK_ARG(num_texels == 0 || (d_positions && d_normals && d_radius && d_out), "NULL argument");
+    return launch_surfel_rays(d_positions, d_normals, d_radius, num_texels, rays_per_texel, tmin, tmax, seed, d_out,
+                              (hipStream_t)stream);
+}
+
+int32_t cwbvh_trace_surfels(const cwbvh_scene* scene, const cwbvh_float4* d_positions, const cwbvh_float4* d_normals,
+                            const float* d_radius, int64_t num_texels, int32_t rays_per_texel, float tmin, float tmax,
+                            uint64_t seed, uint32_t flags, cwbvh_hit* d_hits, uint32_t* d_texel_hits,
+                            const cwbvh_trace_config* cfg, uint64_t* d_counters, void* stream) {
+    CW_CHECK_ARG(scene != nullptr, "scene is NULL");
+    CW_CHECK_ARG(scene->kind == CWBVH_KIND_CWBVH, "surfel traces run the CWBVH kernel");
+    CW_CHECK_ARG(num_texels >= 0 && rays_per_texel >= 1, "bad ray count");
+    const int64_t n = num_texels * (int64_t)rays_per_texel;
+    CW_CHECK_ARG(n <= 0x7FFFFFFF, "more than 2^31 - 1 rays");
+    CW_CHECK_ARG(n == 0 || (d_positions && d_normals && d_radius), "NULL surfel array");
+    CW_CHECK_ARG((d_hits != nullptr) != (d_texel_hits != nullptr), "give exactly one of hits and per-surfel counts");
+    CW_CHECK_ARG((flags & ~(uint32_t)(CWBVH_TRACE_ANY | CWBVH_TRACE_PRIM_ID | CWBVH_TRACE_STATS)) == 0, "unknown flags");
+    CW_CHECK_ARG(!(flags & CWBVH_TRACE_PRIM_ID) || (d_hits && scene->prim_idx), "PRIM_ID needs hits and a prim index");
+    CW_CHECK_ARG(!(flags & CWBVH_TRACE_STATS) || d_counters != nullptr, "STATS needs a counter buffer");
+    cwbvh_trace_config c;
+    if (cfg) c = *cfg;
+    else cwbvh_trace_config_for_scene(scene, &c);
+    int32_t rc = validate_config(scene->kind, c);
+    if (rc) return rc;
+    if (n == 0) return CWBVH_OK;
+    TraceLaunch L{};
+    L.scene = scene;
+    L.hits = d_hits;
+    L.n = n;
+    L.flags = flags;
+    L.cfg = c;
+    L.stats = d_counters;
+    L.stream = (hipStream_t)stream;
+    L.surf_pos = d_positions;
+    L.surf_nrm = d_normals;
+    L.surf_rad = d_radius;
+    L.rays_per_texel = rays_per_texel;
+    L.gen_tmin = tmin;
+    L.gen_tmax = tmax;
+    L.seed = seed;
+    L.texel_hits = d_texel_hits;
+    rc = stream_counters(L.stream, &L.counters);
+    if (rc) return rc;
+    return launch_cwbvh_surfels(L);
+}
+
 int32_t cwbvh_trace_ordered(const cwbvh_scene* scene, const cwbvh_ray* d_rays, const int32_t* d_order,
                             cwbvh_hit* d_hits, int64_t n, uint32_t flags, const cwbvh_trace_config* cfg,
                             uint64_t* d_counters, void* stream) {

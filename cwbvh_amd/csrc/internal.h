@@ -67,9 +67,22 @@ struct TraceLaunch {
     uint64_t* stats;
     unsigned* counters;   // from stream_counters (or a dedicated buffer)
     hipStream_t stream;
+    // rays generated from surfels instead of read (rays == nullptr): ray r is
+    // sample r % rays_per_texel of surfel r / rays_per_texel (aogen.h)
+    const cwbvh_float4* surf_pos = nullptr;
+    const cwbvh_float4* surf_nrm = nullptr;
+    const float* surf_rad = nullptr;
+    int32_t rays_per_texel = 0;
+    float gen_tmin = 0.f, gen_tmax = 0.f;
+    uint64_t seed = 0;
+    uint32_t* texel_hits = nullptr;   // per-surfel hit counts instead of hits
 };
 
 int32_t launch_cwbvh(const TraceLaunch& L);
+int32_t launch_cwbvh_surfels(const TraceLaunch& L);
+int32_t launch_surfel_rays(const cwbvh_float4* pos, const cwbvh_float4* nrm, const float* rad, int64_t num_texels,
+                           int32_t rays_per_texel, float tmin, float tmax, uint64_t seed, cwbvh_ray* out,
+                           hipStream_t stream);
 int32_t launch_bvh2(const TraceLaunch& L);
 
 // Fill the device grid for a persistent kernel: resident blocks per CU * CUs.

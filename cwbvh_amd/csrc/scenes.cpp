@@ -10,6 +10,7 @@
 #include <map>
 #include <unordered_map>
 
+#include "aogen.h"
 #include "common.h"
 
 using namespace cw;
@@ -394,21 +395,17 @@ int32_t cwbvh_rays_ao_from_samples(const cwbvh_float4* positions, const cwbvh_fl
                                    cwbvh_ray* out) {
     CW_CHECK_ARG(num_texels >= 0 && rays_per_texel >= 1, "bad ray count");
     CW_CHECK_ARG(num_texels == 0 || (positions && normals && radius && out), "NULL argument");
+    // the same arithmetic as the device generator (aogen.h), so host and
+    // device rays are bit-identical
     parallel_for(num_texels, num_threads_or_default(0), [&](int64_t b, int64_t e) {
         for (int64_t i = b; i < e; ++i) {
-            // frame of src/ValidationKernels.cu:84-88 (the normal is used as stored)
-            const V3 n{normals[i].x, normals[i].y, normals[i].z};
-            V3 t1 = cross(n, V3{0.f, 0.f, 1.f});
-            if (length(t1) < 0.1f) t1 = cross(n, V3{0.f, 1.f, 0.f});
-            t1 = normalize(t1);
-            const V3 t2 = normalize(cross(t1, n));
-            const V3 o = V3{positions[i].x, positions[i].y, positions[i].z} + n * (radius[i] * 0.5f);
+            const cwao::F3 p{positions[i].x, positions[i].y, positions[i].z};
+            const cwao::F3 n{normals[i].x, normals[i].y, normals[i].z};
             for (int32_t k = 0; k < rays_per_texel; ++k) {
                 const int64_t r = i * rays_per_texel + k;
-                Rng rr(seed, (uint64_t)r);
-                const float u1 = rr.uniform(), u2 = rr.uniform();
-                const V3 l = cosine_hemisphere(u1, u2);
-                put_ray(out[r], o, tmin, normalize(t1 * l.x + t2 * l.y + n * l.z), tmax);
+                cwao::F3 o, d;
+                cwao::surfel_ray(p, n, radius[i], seed, (uint64_t)r, o, d);
+                put_ray(out[r], V3{o.x, o.y, o.z}, tmin, V3{d.x, d.y, d.z}, tmax);
             }
         }
     });

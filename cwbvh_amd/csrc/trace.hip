@@ -30,6 +30,7 @@
 #include <mutex>
 #include <tuple>
 
+#include "aogen.h"
 #include "device_common.h"
 #include "internal.h"
 
@@ -70,7 +71,18 @@ struct CwbvhParams {
     int total_waves;
     int total_blocks;
     int spec_limit;                 // single scene: speculation_limit of its tree
+    // SRC = kRaysFromSurfels: generate ray r from surfel r / rays_per_texel
+    const float4* __restrict__ surf_pos;
+    const float4* __restrict__ surf_nrm;
+    const float* __restrict__ surf_rad;
+    int rays_per_texel;
+    float gen_tmin, gen_tmax;
+    unsigned long long seed;
+    unsigned* texel_hits;           // non-null: count hits per surfel instead of writing hits
 };
+
+constexpr int kRaysFromBuffer = 0;
+constexpr int kRaysFromSurfels = 1;
 
 __device__ __forceinline__ const float4* node_at(const float4* nodes, unsigned i, unsigned stride_sh) {
     return nodes + ((i << 2) + (i << stride_sh));
@@ -134,7 +146,7 @@ constexpr int kLoopSpeculate = 2;
 #define CWBVH_TRACE_ATTR
 #endif
 
-template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP>
+template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer>
 __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhParams P) {
     extern __shared__ unsigned long long lds_stack_raw[];
     CWD_LDS unsigned long long* lds_stack = lds_ptr(lds_stack_raw);
@@ -144,6 +156,7 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
 
     // per-lane ray state
     int out_idx = -1;
+    unsigned out_texel = 0;
     const float4* nodes = P.nodes;
     const float4* tris = P.tris;
     float4* hits = P.hits;
@@ -280,15 +293,30 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                 rays = (const Ray4*)J.rays;
                 order = J.order;
             }
-            const int ri = order ? order[my] : (int)my;
-            out_idx = ri;
-            // rays are read once: keep them out of the caches the BVH lives in
-            const float4 ro = nt_load4(&rays[ri].o);
-            const float4 rd = nt_load4(&rays[ri].d);
-            o = make_float3(ro.x, ro.y, ro.z);
-            d = make_float3(rd.x, rd.y, rd.z);
-            tmin = ro.w;
-            tmax = rd.w;
+            if constexpr (SRC == kRaysFromSurfels) {
+                // the ray is made here from its surfel instead of read back
+                const int ri = (int)my;
+                out_idx = ri;
+                out_texel = (unsigned)ri / (unsigned)P.rays_per_texel;
+                const float4 sp = P.surf_pos[out_texel], sn = P.surf_nrm[out_texel];
+                cwao::F3 go, gd;
+                cwao::surfel_ray(cwao::F3{sp.x, sp.y, sp.z}, cwao::F3{sn.x, sn.y, sn.z}, P.surf_rad[out_texel],
+                                 P.seed, (unsigned long long)ri, go, gd);
+                o = make_float3(go.x, go.y, go.z);
+                d = make_float3(gd.x, gd.y, gd.z);
+                tmin = P.gen_tmin;
+                tmax = P.gen_tmax;
+            } else {
+                const int ri = order ? order[my] : (int)my;
+                out_idx = ri;
+                // rays are read once: keep them out of the caches the BVH lives in
+                const float4 ro = nt_load4(&rays[ri].o);
+                const float4 rd = nt_load4(&rays[ri].d);
+                o = make_float3(ro.x, ro.y, ro.z);
+                d = make_float3(rd.x, rd.y, rd.z);
+                tmin = ro.w;
+                tmax = rd.w;
+            }
             idx = safe_rcp(d.x);
             idy = safe_rcp(d.y);
             idz = safe_rcp(d.z);
@@ -479,8 +507,12 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                 }
             }
             if (finished) {
-                const int out_id = (hit_addr >= 0 && P.prim_id) ? prim_map[hit_addr] : hit_addr;
-                nt_store4(make_float4(tmax, i2f(out_id), hu, hv), &hits[out_idx]);
+                if (SRC == kRaysFromSurfels && P.texel_hits) {
+                    if (hit_addr >= 0) atomicAdd(&P.texel_hits[out_texel], 1u);
+                } else {
+                    const int out_id = (hit_addr >= 0 && P.prim_id) ? prim_map[hit_addr] : hit_addr;
+                    nt_store4(make_float4(tmax, i2f(out_id), hu, hv), &hits[out_idx]);
+                }
                 sp = 0;
                 ng = make_uint2(0u, 0u);
                 tg = make_uint2(0u, 0u);
@@ -666,11 +698,11 @@ int resident_blocks(K kernel, int block, size_t lds) {
     return nb;
 }
 
-template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP>
+template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer>
 int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t stream) {
     const int block = cfg.block_size;
     const size_t lds = (size_t)block * (size_t)cfg.lds_depth * sizeof(unsigned long long);
-    auto kernel = cwbvh_trace<ANY, STATS, MULTI, PREFETCH, LOOP>;
+    auto kernel = cwbvh_trace<ANY, STATS, MULTI, PREFETCH, LOOP, SRC>;
     long long grid = cfg.grid_blocks;
     if (!MULTI && !p.persistent) {
         grid = ((long long)p.n + block - 1) / block;
@@ -721,6 +753,29 @@ int32_t dispatch_cwbvh(CwbvhParams& p, uint32_t flags, const cwbvh_trace_config&
     const bool any = (flags & CWBVH_TRACE_ANY) != 0, stats = (flags & CWBVH_TRACE_STATS) != 0;
     if (any) return stats ? dispatch_prefetch<true, true, MULTI>(p, cfg, st) : dispatch_prefetch<true, false, MULTI>(p, cfg, st);
     return stats ? dispatch_prefetch<false, true, MULTI>(p, cfg, st) : dispatch_prefetch<false, false, MULTI>(p, cfg, st);
+}
+
+template <bool ANY, bool STATS>
+int32_t dispatch_surfels(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t st) {
+    switch (cfg.loop_mode) {
+        case kLoopInterleave: return launch_cwbvh_t<ANY, STATS, false, false, kLoopInterleave, kRaysFromSurfels>(p, cfg, st);
+        case kLoopSpeculate: return launch_cwbvh_t<ANY, STATS, false, false, kLoopSpeculate, kRaysFromSurfels>(p, cfg, st);
+        default: return launch_cwbvh_t<ANY, STATS, false, false, kLoopGroup, kRaysFromSurfels>(p, cfg, st);
+    }
+}
+
+// One thread per ray: the unfused generator (same arithmetic as the fused one).
+__global__ void surfel_rays_kernel(const float4* __restrict__ pos, const float4* __restrict__ nrm,
+                                   const float* __restrict__ rad, long long n_rays, int rays_per_texel, float tmin,
+                                   float tmax, unsigned long long seed, Ray4* __restrict__ out) {
+    const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rays) return;
+    const long long t = r / rays_per_texel;
+    const float4 p = pos[t], n = nrm[t];
+    cwao::F3 o, d;
+    cwao::surfel_ray(cwao::F3{p.x, p.y, p.z}, cwao::F3{n.x, n.y, n.z}, rad[t], seed, (unsigned long long)r, o, d);
+    nt_store4(make_float4(o.x, o.y, o.z, tmin), &out[r].o);
+    nt_store4(make_float4(d.x, d.y, d.z, tmax), &out[r].d);
 }
 
 }  // namespace
@@ -787,6 +842,58 @@ int32_t launch_cwbvh(const TraceLaunch& L) {
     p.persistent = L.cfg.persistent == 2 ? (L.n >= cw::kAutoPersistentRays ? 1 : 0) : L.cfg.persistent;
     p.spec_limit = cw::speculation_limit(s->stack_need);
     return dispatch_cwbvh<false>(p, L.flags, L.cfg, L.stream);
+}
+
+int32_t launch_cwbvh_surfels(const TraceLaunch& L) {
+    if (L.cfg.prefetch != 0) {
+        set_error("surfel traces do not use node prefetch");
+        return CWBVH_ERR_INVALID_ARGUMENT;
+    }
+    const cwbvh_scene* s = L.scene;
+    CwbvhParams p{};
+    p.nodes = (const float4*)s->nodes;
+    p.tris = (const float4*)s->woop;
+    p.prim_map = s->prim_idx;
+    p.rays = nullptr;
+    p.order = nullptr;
+    p.hits = (float4*)L.hits;
+    p.work = L.counters;
+    p.stats = (unsigned long long*)L.stats;
+    p.n = (int)L.n;
+    p.stride = s->node_stride;
+    p.lds_depth = L.cfg.lds_depth;
+    p.chunk = L.cfg.fetch_chunk;
+    p.idle_lanes = L.cfg.refill_idle;
+    p.idle_budget = L.cfg.refill_budget;
+    p.postpone_pct = L.cfg.postpone_pct;
+    p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
+    p.slices = L.cfg.work_slices;
+    p.persistent = L.cfg.persistent == 2 ? (L.n >= cw::kAutoPersistentRays ? 1 : 0) : L.cfg.persistent;
+    p.spec_limit = cw::speculation_limit(s->stack_need);
+    p.surf_pos = (const float4*)L.surf_pos;
+    p.surf_nrm = (const float4*)L.surf_nrm;
+    p.surf_rad = L.surf_rad;
+    p.rays_per_texel = L.rays_per_texel;
+    p.gen_tmin = L.gen_tmin;
+    p.gen_tmax = L.gen_tmax;
+    p.seed = L.seed;
+    p.texel_hits = L.texel_hits;
+    const bool any = (L.flags & CWBVH_TRACE_ANY) != 0, stats = (L.flags & CWBVH_TRACE_STATS) != 0;
+    if (any) return stats ? dispatch_surfels<true, true>(p, L.cfg, L.stream) : dispatch_surfels<true, false>(p, L.cfg, L.stream);
+    return stats ? dispatch_surfels<false, true>(p, L.cfg, L.stream) : dispatch_surfels<false, false>(p, L.cfg, L.stream);
+}
+
+int32_t launch_surfel_rays(const cwbvh_float4* pos, const cwbvh_float4* nrm, const float* rad, int64_t num_texels,
+                           int32_t rays_per_texel, float tmin, float tmax, uint64_t seed, cwbvh_ray* out,
+                           hipStream_t stream) {
+    const long long n = (long long)num_texels * rays_per_texel;
+    if (n == 0) return CWBVH_OK;
+    const int block = 256;
+    hipLaunchKernelGGL(surfel_rays_kernel, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0, stream,
+                       (const float4*)pos, (const float4*)nrm, rad, n, rays_per_texel, tmin, tmax,
+                       (unsigned long long)seed, (Ray4*)out);
+    CW_HIP(hipGetLastError());
+    return CWBVH_OK;
 }
 
 int32_t launch_cwbvh_batch(const BatchLaunch& L) {

@@ -166,10 +166,12 @@ int32_t cwbvh_rays_ao_from_hits(const cwbvh_mesh* mesh, const cwbvh_ray* primary
  * (src/ValidationKernels.cu:63-101): for texel i and k < rays_per_texel, a
  * cosine-weighted direction (concentric disk lifted to the hemisphere) in the
  * frame t1 = normalize(n x z) (n x y when that is shorter than 0.1),
- * t2 = normalize(t1 x n), and origin p + n * radius * 0.5.  The reference
+ * t2 = normalize(t1 x n), and origin p + (n * radius) * 0.5.  The reference
  * uses tmin 0.01, tmax 1e20 and curand; the random stream here is the
  * library's own (seeded), so the same distribution, not the same rays.
- * out[i * rays_per_texel + k]. */
+ * out[i * rays_per_texel + k].  The host generator, the device generator
+ * (cwbvh_rays_ao_from_samples_device) and the fused trace (cwbvh_trace_surfels)
+ * produce the same bits. */
 int32_t cwbvh_rays_ao_from_samples(const cwbvh_float4* positions, const cwbvh_float4* normals,
                                    const float* radius, int64_t num_texels, int32_t rays_per_texel,
                                    float tmin, float tmax, uint64_t seed, cwbvh_ray* out);
@@ -332,6 +334,24 @@ int32_t cwbvh_trace(const cwbvh_scene* scene, const cwbvh_ray* d_rays, cwbvh_hit
 int32_t cwbvh_trace_ordered(const cwbvh_scene* scene, const cwbvh_ray* d_rays,
                             const int32_t* d_order, cwbvh_hit* d_hits, int64_t n,
                             uint32_t flags, const cwbvh_trace_config* cfg,
+                            uint64_t* d_counters, void* stream);
+
+/* Device generator of cwbvh_rays_ao_from_samples (same rays, bit for bit):
+ * surfel arrays in device memory, rays written to d_out on `stream`. */
+int32_t cwbvh_rays_ao_from_samples_device(const cwbvh_float4* d_positions, const cwbvh_float4* d_normals,
+                                          const float* d_radius, int64_t num_texels, int32_t rays_per_texel,
+                                          float tmin, float tmax, uint64_t seed, cwbvh_ray* d_out, void* stream);
+/* AO rays generated inside the CWBVH traversal kernel (no ray buffer): ray
+ * r = i * rays_per_texel + k is the ray cwbvh_rays_ao_from_samples makes for
+ * surfel i, sample k.  Output is either one hit per ray in d_hits (as
+ * cwbvh_trace writes them for the generated rays) or, with d_hits NULL, the
+ * number of rays of each surfel that hit something added to d_texel_hits[i]
+ * (uint32 per surfel; the caller clears it).  Flags as cwbvh_trace (PRIM_ID
+ * only with d_hits); node prefetch is not used. */
+int32_t cwbvh_trace_surfels(const cwbvh_scene* scene, const cwbvh_float4* d_positions,
+                            const cwbvh_float4* d_normals, const float* d_radius, int64_t num_texels,
+                            int32_t rays_per_texel, float tmin, float tmax, uint64_t seed, uint32_t flags,
+                            cwbvh_hit* d_hits, uint32_t* d_texel_hits, const cwbvh_trace_config* cfg,
                             uint64_t* d_counters, void* stream);
 
 /* 1 when a traversal on this stream overflowed its stack since the last
