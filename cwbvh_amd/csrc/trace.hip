@@ -198,7 +198,7 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
     unsigned pool_next = 0, pool_end = 0;
     int pool_job = 0;
     bool exhausted = false;
-    const int home = MULTI ? 0 : (int)(blockIdx.x % (unsigned)P.slices);
+    const int home = (int)(blockIdx.x % (unsigned)P.slices);
     int slice_k = 0;
 
     while (true) {
@@ -226,10 +226,25 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                         break;
                     }
                 } else if (MULTI) {
+                    // claim one chunk index from the home slice of the chunk
+                    // range, then from the others
+                    bool found = false;
                     unsigned base = 0;
-                    if (lane == leader) base = atomicAdd(&P.work[0], 1u);
-                    base = __shfl(base, leader);
-                    if (base >= (unsigned)P.n) {
+                    while (slice_k < P.slices) {
+                        const int sl = (home + slice_k) % P.slices;
+                        const unsigned lo = (unsigned)(((unsigned long long)P.n * (unsigned)sl) / (unsigned)P.slices);
+                        const unsigned hi = (unsigned)(((unsigned long long)P.n * (unsigned)(sl + 1)) / (unsigned)P.slices);
+                        unsigned claim = 0;
+                        if (lane == leader) claim = atomicAdd(&P.work[sl * kWorkStride], 1u);
+                        claim = __shfl(claim, leader);
+                        if (claim < hi - lo) {
+                            base = lo + claim;
+                            found = true;
+                            break;
+                        }
+                        ++slice_k;
+                    }
+                    if (!found) {
                         exhausted = true;
                         break;
                     }
@@ -913,7 +928,7 @@ int32_t launch_cwbvh_batch(const BatchLaunch& L) {
     p.idle_budget = L.cfg.refill_budget;
     p.postpone_pct = L.cfg.postpone_pct;
     p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
-    p.slices = 1;
+    p.slices = L.cfg.work_slices;
     p.persistent = 1;
     return dispatch_cwbvh<true>(p, L.flags, L.cfg, L.stream);
 }
