@@ -31,8 +31,11 @@
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
+#include <condition_variable>
+#include <memory>
 #include <mutex>
 #include <numeric>
+#include <thread>
 
 #include "common.h"
 
@@ -53,20 +56,35 @@ struct Tree {
     int32_t root = 0;
 };
 
+// A triangle reference with its box and centroid, so the recursive passes
+// scan and partition one contiguous array instead of gathering from two.
+struct PrimRef {
+    Box box;
+    V3 cen;
+    int32_t id;
+};
+
 struct Builder {
-    const std::vector<Box>& pbox;
-    const std::vector<V3>& pcen;
     std::vector<BNode>& nodes;
-    std::vector<int32_t>& refs;
+    std::vector<PrimRef>& refs;
     std::atomic<int32_t> next_node{0};
-    std::atomic<int> live_threads{1};
     int max_threads;
     int bins;
     int max_depth;
 
-    Builder(const std::vector<Box>& b, const std::vector<V3>& c, std::vector<BNode>& n, std::vector<int32_t>& r,
-            int threads, int nbins, int maxd)
-        : pbox(b), pcen(c), nodes(n), refs(r), max_threads(threads), bins(nbins), max_depth(maxd) {}
+    // Subtrees at least this large are handed to the worker pool; smaller
+    // ones are finished by the thread that split them.
+    static constexpr int32_t kTaskMin = 4096;
+    struct Task {
+        int32_t idx, b, e, depth;
+    };
+    std::mutex mu;
+    std::condition_variable cv;
+    std::vector<Task> tasks;
+    int64_t outstanding = 0;
+
+    Builder(std::vector<BNode>& n, std::vector<PrimRef>& r, int threads, int nbins, int maxd)
+        : nodes(n), refs(r), max_threads(threads), bins(nbins), max_depth(maxd) {}
 
     int32_t alloc2() { return next_node.fetch_add(2); }
 
@@ -75,16 +93,25 @@ struct Builder {
         int32_t n = 0;
     };
 
-    // Bin of centroid coordinate x over [lo, hi] split into nb bins, or -1 when
-    // the extent is not a positive finite number (computed in double, so tiny
-    // or huge extents neither overflow nor produce NaN).
-    static int bin_of(float x, float lo, float hi, int nb) {
-        const double ext = (double)hi - (double)lo;
-        if (!(ext > 0.0) || !std::isfinite(ext)) return -1;
-        const double f = ((double)x - (double)lo) / ext * nb;
-        if (!(f > 0.0)) return 0;
-        return f >= nb - 1 ? nb - 1 : (int)f;
-    }
+    // Centroid bins of one axis over [lo, hi]: nb bins of equal width, in
+    // double so that tiny or huge extents neither overflow nor produce NaN.
+    // Binning and partitioning use the same mapping.
+    struct AxisBins {
+        double lo = 0.0, scale = 0.0;
+        int nb = 0;
+        bool ok = false;
+        AxisBins(float l, float h, int n) : nb(n) {
+            const double ext = (double)h - (double)l;
+            ok = ext > 0.0 && std::isfinite(ext);
+            lo = l;
+            scale = ok ? (double)n / ext : 0.0;
+        }
+        int bin(float x) const {
+            const double f = ((double)x - lo) * scale;
+            if (!(f > 0.0)) return 0;
+            return f >= nb - 1 ? nb - 1 : (int)f;
+        }
+    };
 
     // Binned SAH over [b, e).  Returns the partition point or -1 for "no good
     // split" (then the caller splits at the median).
@@ -95,17 +122,19 @@ struct Builder {
         bool force_median = depth + log2n >= max_depth;
         int best_axis = -1, best_bin = -1;
         double best_cost = INFINITY;
+        const int nb = std::min(bins, std::max(4, n));
+        const AxisBins ax[3] = {AxisBins(cb.lo.x, cb.hi.x, nb), AxisBins(cb.lo.y, cb.hi.y, nb),
+                                AxisBins(cb.lo.z, cb.hi.z, nb)};
         if (!force_median) {
-            int nb = std::min(bins, std::max(4, n));
-            std::vector<Bin> bin(3 * nb);
+            thread_local std::vector<Bin> bin;
+            bin.assign(3 * (size_t)nb, Bin());
             auto bin_range = [&](int32_t lo, int32_t hi, std::vector<Bin>& out) {
                 for (int32_t i = lo; i < hi; ++i) {
-                    int32_t p = refs[i];
+                    const PrimRef& r = refs[i];
                     for (int a = 0; a < 3; ++a) {
-                        const int k = bin_of(comp(pcen[p], a), comp(cb.lo, a), comp(cb.hi, a), nb);
-                        if (k < 0) continue;
-                        Bin& bb = out[a * nb + k];
-                        bb.box.grow(pbox[p]);
+                        if (!ax[a].ok) continue;
+                        Bin& bb = out[a * nb + ax[a].bin(comp(r.cen, a))];
+                        bb.box.grow(r.box);
                         bb.n++;
                     }
                 }
@@ -129,9 +158,9 @@ struct Builder {
             } else {
                 bin_range(b, e, bin);
             }
-            std::vector<double> right_cost(nb);
+            double right_cost[257];
             for (int a = 0; a < 3; ++a) {
-                if (!(comp(cb.hi, a) - comp(cb.lo, a) > 0)) continue;
+                if (!ax[a].ok) continue;
                 Box acc;
                 int32_t cnt = 0;
                 for (int k = nb - 1; k >= 1; --k) {
@@ -155,22 +184,63 @@ struct Builder {
             }
         }
         if (best_axis < 0) return -1;
-        int nb = std::min(bins, std::max(4, n));
-        const float lo_a = comp(cb.lo, best_axis), hi_a = comp(cb.hi, best_axis);
-        auto mid = std::partition(refs.begin() + b, refs.begin() + e, [&](int32_t p) {
-            return bin_of(comp(pcen[p], best_axis), lo_a, hi_a, nb) <= best_bin;
+        const AxisBins& A = ax[best_axis];
+        auto mid = std::partition(refs.begin() + b, refs.begin() + e, [&](const PrimRef& r) {
+            return A.bin(comp(r.cen, best_axis)) <= best_bin;
         });
         int32_t m = (int32_t)(mid - refs.begin());
         if (m == b || m == e) return -1;
         return m;
     }
 
+    void enqueue(const Task& t) {
+        {
+            std::lock_guard<std::mutex> g(mu);
+            tasks.push_back(t);
+            ++outstanding;
+        }
+        cv.notify_one();
+    }
+
+    void worker() {
+        for (;;) {
+            Task t;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return !tasks.empty() || outstanding == 0; });
+                if (tasks.empty()) return;
+                t = tasks.back();
+                tasks.pop_back();
+            }
+            build(t.idx, t.b, t.e, t.depth);
+            bool last;
+            {
+                std::lock_guard<std::mutex> g(mu);
+                last = --outstanding == 0;
+            }
+            if (last) cv.notify_all();
+        }
+    }
+
+    // Build the whole tree over refs[0, n) with the worker pool.
+    void run(int32_t n) {
+        next_node = 1;
+        if (max_threads <= 1 || n < 2 * kTaskMin) {
+            build(0, 0, n, 1);
+            return;
+        }
+        enqueue(Task{0, 0, n, 1});
+        std::vector<std::thread> pool;
+        for (int i = 0; i < max_threads; ++i) pool.emplace_back([this] { worker(); });
+        for (auto& th : pool) th.join();
+    }
+
     void build(int32_t idx, int32_t b, int32_t e, int depth) {
         BNode& nd = nodes[idx];
         Box box, cb;
         for (int32_t i = b; i < e; ++i) {
-            box.grow(pbox[refs[i]]);
-            cb.grow(pcen[refs[i]]);
+            box.grow(refs[i].box);
+            cb.grow(refs[i].cen);
         }
         nd.box = box;
         nd.prims = e - b;
@@ -185,47 +255,44 @@ struct Builder {
             V3 d = cb.hi - cb.lo;
             int a = (d.x >= d.y && d.x >= d.z) ? 0 : (d.y >= d.z ? 1 : 2);
             m = b + (e - b) / 2;
-            std::nth_element(refs.begin() + b, refs.begin() + m, refs.begin() + e, [&](int32_t p, int32_t q) {
-                float cp = comp(pcen[p], a), cq = comp(pcen[q], a);
-                return cp < cq || (cp == cq && p < q);
-            });
+            std::nth_element(refs.begin() + b, refs.begin() + m, refs.begin() + e,
+                             [&](const PrimRef& p, const PrimRef& q) {
+                                 const float cp = comp(p.cen, a), cq = comp(q.cen, a);
+                                 return cp < cq || (cp == cq && p.id < q.id);
+                             });
         }
         int32_t c = alloc2();
         nd.left = c;
         nd.right = c + 1;
-        bool spawn = (e - b) > 32768 && live_threads.load() < max_threads;
-        if (spawn) {
-            live_threads.fetch_add(1);
-            std::thread t([this, c, b, m, depth] { build(c, b, m, depth + 1); });
-            build(c + 1, m, e, depth + 1);
-            t.join();
-            live_threads.fetch_sub(1);
-        } else {
-            build(c, b, m, depth + 1);
-            build(c + 1, m, e, depth + 1);
-        }
+        // the larger child goes to the pool when it is worth a task; this
+        // thread continues with the other (the tree does not depend on which
+        // thread builds what, only node numbering does)
+        const bool left_big = (m - b) >= (e - m);
+        const int32_t bb = left_big ? b : m, be = left_big ? m : e, sb = left_big ? m : b, se = left_big ? e : m;
+        const int32_t bidx = left_big ? c : c + 1, sidx = left_big ? c + 1 : c;
+        if (max_threads > 1 && be - bb >= kTaskMin) enqueue(Task{bidx, bb, be, depth + 1});
+        else build(bidx, bb, be, depth + 1);
+        build(sidx, sb, se, depth + 1);
     }
 };
 
 void build_tree(const cwbvh_mesh* mesh, const cwbvh_build_params& P, Tree& T) {
     int64_t nt = mesh->nt();
-    std::vector<Box> pbox(nt);
-    std::vector<V3> pcen(nt);
     int threads = num_threads_or_default(P.num_threads);
+    std::vector<PrimRef> refs((size_t)nt);
     parallel_for(nt, threads, [&](int64_t b, int64_t e) {
         for (int64_t t = b; t < e; ++t) {
             V3 a, bb, c;
             mesh->tri(t, a, bb, c);
-            Box x;
-            x.grow(a);
-            x.grow(bb);
-            x.grow(c);
-            pbox[t] = x;
-            pcen[t] = x.center();
+            PrimRef& r = refs[(size_t)t];
+            r.box = Box();
+            r.box.grow(a);
+            r.box.grow(bb);
+            r.box.grow(c);
+            r.cen = r.box.center();
+            r.id = (int32_t)t;
         }
     });
-    T.refs.resize(nt);
-    std::iota(T.refs.begin(), T.refs.end(), 0);
     T.nodes.assign(std::max<int64_t>(1, 2 * nt - 1), BNode());
     T.root = 0;
     if (nt == 0) {
@@ -234,10 +301,13 @@ void build_tree(const cwbvh_mesh* mesh, const cwbvh_build_params& P, Tree& T) {
         T.nodes[0].prims = 0;
         return;
     }
-    Builder B(pbox, pcen, T.nodes, T.refs, threads, P.sah_bins, P.max_depth);
-    B.next_node = 1;
-    B.build(0, 0, (int32_t)nt, 1);
+    Builder B(T.nodes, refs, threads, P.sah_bins, P.max_depth);
+    B.run((int32_t)nt);
     T.nodes.resize(B.next_node.load());
+    T.refs.resize((size_t)nt);
+    parallel_for(nt, threads, [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) T.refs[(size_t)i] = refs[(size_t)i].id;
+    });
 }
 
 // ------------------------------------------------------------------ Woop
@@ -270,24 +340,55 @@ void woopify(V3 v0, V3 v1, V3 v2, cwbvh_float4 out[3]) {
     if (f2u(out[0].x) == 0x80000000u) out[0].x = 0.0f;
 }
 
-void append_tri(const cwbvh_mesh* mesh, int32_t prim, cwbvh_layout& L) {
-    V3 a, b, c;
-    mesh->tri(prim, a, b, c);
-    cwbvh_float4 w[3];
-    woopify(a, b, c, w);
-    L.woop.push_back(w[0]);
-    L.woop.push_back(w[1]);
-    L.woop.push_back(w[2]);
+// A triangle whose three Woop rows start at woop[addr].  The layout passes
+// only reserve the rows; fill_woop computes all of them in parallel once the
+// structure is known.
+struct TriSlot {
+    int64_t addr;
+    int32_t prim;
+};
+
+void reserve_tri(int32_t prim, cwbvh_layout& L, std::vector<TriSlot>& slots) {
+    slots.push_back(TriSlot{(int64_t)L.woop.size(), prim});
+    L.woop.resize(L.woop.size() + 3);
     L.prim_idx.push_back(prim);
     L.prim_idx.push_back(-1);
     L.prim_idx.push_back(-1);
     L.num_tri_refs++;
 }
 
+void fill_woop(const cwbvh_mesh* mesh, const std::vector<TriSlot>& slots, int threads, cwbvh_layout& L) {
+    parallel_for((int64_t)slots.size(), threads, [&](int64_t b, int64_t e) {
+        for (int64_t i = b; i < e; ++i) {
+            V3 a, bb, c;
+            mesh->tri(slots[(size_t)i].prim, a, bb, c);
+            woopify(a, bb, c, &L.woop[(size_t)slots[(size_t)i].addr]);
+        }
+    });
+}
+
+// Triangle ids under tree node n, left to right; `st` is scratch.
+void gather_prims(const Tree& T, int32_t n, std::vector<int32_t>& st, std::vector<int32_t>& out) {
+    st.clear();
+    st.push_back(n);
+    while (!st.empty()) {
+        const int32_t x = st.back();
+        st.pop_back();
+        const BNode& b = T.nodes[x];
+        if (b.left < 0) {
+            for (int32_t k = 0; k < b.count; ++k) out.push_back(T.refs[b.first + k]);
+        } else {
+            st.push_back(b.right);
+            st.push_back(b.left);
+        }
+    }
+}
+
 // ------------------------------------------------------------------ BVH2
 
 void layout_bvh2(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params& P, cwbvh_layout& L) {
     const auto& N = T.nodes;
+    const int threads = num_threads_or_default(P.num_threads);
     int32_t nn = (int32_t)N.size();
     // bottom-up SAH: make a leaf wherever it is cheaper (children indices > parent)
     std::vector<double> cost(nn);
@@ -314,22 +415,19 @@ void layout_bvh2(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params
 
     L.kind = CWBVH_KIND_BVH2;
     L.node_stride = 4;
+    const int64_t nt = (int64_t)T.refs.size();
+    std::vector<TriSlot> slots;
+    slots.reserve((size_t)nt);
+    L.woop.reserve((size_t)(4 * nt + 2));
+    L.prim_idx.reserve((size_t)(4 * nt + 2));
+    std::vector<int32_t> scratch, prims;
     auto gather = [&](int32_t n, std::vector<int32_t>& out) {
-        std::vector<int32_t> st{n};
-        while (!st.empty()) {
-            int32_t x = st.back();
-            st.pop_back();
-            if (N[x].left < 0) {
-                for (int32_t k = 0; k < N[x].count; ++k) out.push_back(T.refs[N[x].first + k]);
-            } else {
-                st.push_back(N[x].right);
-                st.push_back(N[x].left);
-            }
-        }
+        out.clear();
+        gather_prims(T, n, scratch, out);
     };
     auto emit_leaf = [&](const std::vector<int32_t>& prims) -> int32_t {
         int32_t addr = (int32_t)L.woop.size();
-        for (int32_t p : prims) append_tri(mesh, p, L);
+        for (int32_t p : prims) reserve_tri(p, L, slots);
         L.woop.push_back(cwbvh_float4{u2f(0x80000000u), 0.f, 0.f, 0.f});
         L.prim_idx.push_back(-1);
         L.num_leaves++;
@@ -364,12 +462,12 @@ void layout_bvh2(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params
     if (root.prims == 0 || leaf[T.root]) {
         // wrap: inner root whose second child can never be hit
         int32_t r = new_node();
-        std::vector<int32_t> prims;
         if (root.prims) gather(T.root, prims);
         put_box(r, 0, root.prims ? root.box : inverted);
         put_child(r, 0, emit_leaf(prims));
         put_box(r, 1, inverted);
         put_child(r, 1, emit_leaf({}));
+        fill_woop(mesh, slots, threads, L);
         L.max_depth = 1;
         L.stack_need = 2;
         return;
@@ -383,7 +481,6 @@ void layout_bvh2(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params
         st.pop_back();
         const BNode& nd = N[it.n];
         if (leaf[it.n]) {
-            std::vector<int32_t> prims;
             gather(it.n, prims);
             put_child(it.parent, it.slot, emit_leaf(prims));
             continue;
@@ -396,6 +493,7 @@ void layout_bvh2(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params
         st.push_back({nd.right, a, 1, it.depth + 1});
         st.push_back({nd.left, a, 0, it.depth + 1});
     }
+    fill_woop(mesh, slots, threads, L);
     L.max_depth = maxd;
     L.stack_need = maxd + 1;
 }
@@ -512,22 +610,45 @@ void assign_slots(const Box& nb, const std::vector<Box>& cbox, int slot_of[8]) {
     int nc = (int)cbox.size();
     V3 nc_c = nb.center();
     struct Cand { float cost; int s, c; };
-    std::vector<Cand> cand;
+    // at most 8 slots x 8 children, sorted by cost with ties kept in
+    // (slot, child) order (insertion sort is stable)
+    Cand cand[64];
+    int ncand = 0;
     for (int s = 0; s < 8; ++s) {
         V3 ds{(s & 4) ? -1.f : 1.f, (s & 2) ? -1.f : 1.f, (s & 1) ? -1.f : 1.f};
-        for (int c = 0; c < nc; ++c) cand.push_back({dot(cbox[c].center() - nc_c, ds), s, c});
+        for (int c = 0; c < nc; ++c) {
+            const Cand x{dot(cbox[c].center() - nc_c, ds), s, c};
+            int j = ncand++;
+            while (j > 0 && x.cost < cand[j - 1].cost) {
+                cand[j] = cand[j - 1];
+                --j;
+            }
+            cand[j] = x;
+        }
     }
-    std::stable_sort(cand.begin(), cand.end(), [](const Cand& a, const Cand& b) { return a.cost < b.cost; });
     bool used[8] = {};
     for (int c = 0; c < nc; ++c) slot_of[c] = -1;
     int assigned = 0;
-    for (const Cand& x : cand) {
-        if (assigned == nc) break;
+    for (int i = 0; i < ncand && assigned < nc; ++i) {
+        const Cand& x = cand[i];
         if (used[x.s] || slot_of[x.c] >= 0) continue;
         used[x.s] = true;
         slot_of[x.c] = x.s;
         ++assigned;
     }
+}
+
+// floor and ceil of a child's offset in quantization steps, clamped to the
+// byte range (NaN gives 0)
+inline int quant_floor(double x) {
+    if (!(x > 0.0)) return 0;
+    return x >= 255.0 ? 255 : (int)x;
+}
+inline int quant_ceil(double x) {
+    if (!(x > 0.0)) return 0;
+    if (x >= 255.0) return 255;
+    const int t = (int)x;
+    return t + ((double)t < x ? 1 : 0);
 }
 
 void layout_cwbvh(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params& P, cwbvh_layout& L) {
@@ -542,20 +663,13 @@ void layout_cwbvh(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_param
         L.num_nodes += count;
         return idx;
     };
-    auto gather = [&](int32_t n, std::vector<int32_t>& out) {
-        std::vector<int32_t> st{n};
-        while (!st.empty()) {
-            int32_t x = st.back();
-            st.pop_back();
-            const BNode& b = T.nodes[x];
-            if (b.left < 0) {
-                for (int32_t k = 0; k < b.count; ++k) out.push_back(T.refs[b.first + k]);
-            } else {
-                st.push_back(b.right);
-                st.push_back(b.left);
-            }
-        }
-    };
+    const int threads = num_threads_or_default(P.num_threads);
+    const int64_t nt = (int64_t)T.refs.size();
+    std::vector<TriSlot> slots;
+    slots.reserve((size_t)nt);
+    L.woop.reserve((size_t)(3 * nt));
+    L.prim_idx.reserve((size_t)(3 * nt));
+    std::vector<int32_t> scratch, prims;
     double sah = 0.0;
     double root_area = T.nodes.empty() ? 0.0 : T.nodes[T.root].box.area();
 
@@ -610,18 +724,15 @@ void layout_cwbvh(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_param
         node[0].x = nb.lo.x;
         node[0].y = nb.lo.y;
         node[0].z = nb.lo.z;
+        double scale[3];
+        for (int a = 0; a < 3; ++a) scale[a] = std::ldexp(1.0, -ex[a]);
         for (int s = 0; s < 8; ++s) {
             int c = child_at[s];
             if (c < 0) continue;
             const Box& b = cbox[c];
             for (int a = 0; a < 3; ++a) {
-                double scale = std::ldexp(1.0, -ex[a]);
-                double lo = std::floor(((double)comp(b.lo, a) - (double)comp(nb.lo, a)) * scale);
-                double hi = std::ceil(((double)comp(b.hi, a) - (double)comp(nb.lo, a)) * scale);
-                lo = std::min(255.0, std::max(0.0, lo));
-                hi = std::min(255.0, std::max(0.0, hi));
-                q[a * 8 + s] = (uint8_t)lo;
-                q[24 + a * 8 + s] = (uint8_t)hi;
+                q[a * 8 + s] = (uint8_t)quant_floor(((double)comp(b.lo, a) - (double)comp(nb.lo, a)) * scale[a]);
+                q[24 + a * 8 + s] = (uint8_t)quant_ceil(((double)comp(b.hi, a) - (double)comp(nb.lo, a)) * scale[a]);
             }
             int32_t bn = kids[c];
             if (!CL.is_leaf(bn)) {
@@ -630,13 +741,13 @@ void layout_cwbvh(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_param
                 st.push_back({bn, child_base + inner_count, it.depth + 1});
                 ++inner_count;
             } else {
-                std::vector<int32_t> prims;
-                gather(bn, prims);
+                prims.clear();
+                gather_prims(T, bn, scratch, prims);
                 int cnt = (int)prims.size();
                 unsigned unary = cnt == 1 ? 1u : (cnt == 2 ? 3u : 7u);
                 meta[s] = (uint8_t)((unary << 5) | (unsigned)tri_count);
                 tri_count += cnt;
-                for (int32_t p : prims) append_tri(mesh, p, L);
+                for (int32_t p : prims) reserve_tri(p, L, slots);
                 L.num_leaves++;
                 if (root_area > 0) sah += P.cwbvh_tri_cost * cnt * b.area() / root_area;
             }
@@ -648,6 +759,7 @@ void layout_cwbvh(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_param
         node[1].x = i2f(child_base);
         node[1].y = i2f(tri_base);
     }
+    fill_woop(mesh, slots, threads, L);
     L.max_depth = maxd;
     L.stack_need = maxd + 1;
     L.sah_cost = sah;
@@ -695,16 +807,20 @@ int32_t cwbvh_build_layouts(const cwbvh_mesh* mesh, const cwbvh_build_params* pa
     CW_CHECK_ARG(mesh->nt() < (int64_t)0x7FFFFFFF / 4, "too many triangles for 32-bit layouts");
     Tree T;
     build_tree(mesh, P, T);
-    if (bvh2_out) {
-        auto* L = new cwbvh_layout();
-        layout_bvh2(mesh, T, P, *L);
-        *bvh2_out = L;
+    std::unique_ptr<cwbvh_layout> bvh2(bvh2_out ? new cwbvh_layout() : nullptr);
+    std::unique_ptr<cwbvh_layout> cwb(cwbvh_out ? new cwbvh_layout() : nullptr);
+    // the two layouts only read the tree: with both requested and more than
+    // one thread, the BVH2 one runs beside the CWBVH one
+    if (bvh2 && cwb && num_threads_or_default(P.num_threads) > 1) {
+        std::thread side([&] { layout_bvh2(mesh, T, P, *bvh2); });
+        layout_cwbvh(mesh, T, P, *cwb);
+        side.join();
+    } else {
+        if (bvh2) layout_bvh2(mesh, T, P, *bvh2);
+        if (cwb) layout_cwbvh(mesh, T, P, *cwb);
     }
-    if (cwbvh_out) {
-        auto* L = new cwbvh_layout();
-        layout_cwbvh(mesh, T, P, *L);
-        *cwbvh_out = L;
-    }
+    if (bvh2_out) *bvh2_out = bvh2.release();
+    if (cwbvh_out) *cwbvh_out = cwb.release();
     return CWBVH_OK;
 }
 

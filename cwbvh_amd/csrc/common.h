@@ -44,8 +44,13 @@ inline V3 normalize(V3 a) {
     float l = length(a);
     return l > 0.f ? a * (1.0f / l) : a;
 }
-inline V3 vmin(V3 a, V3 b) { return {std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)}; }
-inline V3 vmax(V3 a, V3 b) { return {std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)}; }
+// Component-wise min/max that keep `a` when `b` is NaN (as fmin/fmax do for a
+// NaN operand), written as compares so that they compile to minss/maxss
+// instead of calls into libm; callers pass the accumulator as `a`.
+inline float fmin_acc(float a, float b) { return b < a ? b : a; }
+inline float fmax_acc(float a, float b) { return b > a ? b : a; }
+inline V3 vmin(V3 a, V3 b) { return {fmin_acc(a.x, b.x), fmin_acc(a.y, b.y), fmin_acc(a.z, b.z)}; }
+inline V3 vmax(V3 a, V3 b) { return {fmax_acc(a.x, b.x), fmax_acc(a.y, b.y), fmax_acc(a.z, b.z)}; }
 inline float comp(V3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
 
 struct Box {

@@ -185,6 +185,21 @@ def test_depth_limit_respected(cw):
     assert cwb.stats()["stack_need"] <= 21
 
 
+def test_build_same_for_any_thread_count(cw):
+    """The parallel builder (work queue over subtrees, parallel binning of
+    large nodes, Woop rows in parallel, both layouts side by side) returns
+    the same bytes as the single-threaded one."""
+    mesh = cw.Mesh.cluster_soup(300_000, 16, 0.002, 0.2, -5.0, 0.5, seed=8)
+    ref = cw.build_layouts(mesh, cw.default_build_params(num_threads=1))
+    for t in (3, 8):
+        got = cw.build_layouts(mesh, cw.default_build_params(num_threads=t))
+        for a, b in zip(ref, got):
+            assert np.array_equal(a.nodes.view(np.uint32), b.nodes.view(np.uint32))
+            assert np.array_equal(a.woop.view(np.uint32), b.woop.view(np.uint32))
+            assert np.array_equal(a.prim_idx, b.prim_idx)
+            assert a.stats() == b.stats()
+
+
 def test_sah_quality_cwbvh_fewer_bytes(cw, orc):
     """The 8-wide collapse must fetch fewer node bytes per ray than BVH2 on C0."""
     from cwbvh_amd import workloads as W
