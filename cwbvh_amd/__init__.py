@@ -52,6 +52,8 @@ class BuildParams(C.Structure):
         ("cwbvh_tri_cost", C.c_float),
         ("cwbvh_node_stride", C.c_int32),
         ("max_depth", C.c_int32),
+        ("split_alpha", C.c_float),
+        ("split_budget", C.c_float),
     ]
 
 

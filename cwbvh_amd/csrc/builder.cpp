@@ -58,39 +58,132 @@ struct Tree {
 
 // A triangle reference with its box and centroid, so the recursive passes
 // scan and partition one contiguous array instead of gathering from two.
+// With spatial splits a triangle can have several references, each with the
+// box of the part of the triangle inside its subtree.
 struct PrimRef {
     Box box;
     V3 cen;
     int32_t id;
 };
 
+bool box_empty(const Box& b) { return !(b.lo.x <= b.hi.x && b.lo.y <= b.hi.y && b.lo.z <= b.hi.z); }
+
+Box box_and(const Box& a, const Box& b) {
+    Box r;
+    r.lo = V3{std::max(a.lo.x, b.lo.x), std::max(a.lo.y, b.lo.y), std::max(a.lo.z, b.lo.z)};
+    r.hi = V3{std::min(a.hi.x, b.hi.x), std::min(a.hi.y, b.hi.y), std::min(a.hi.z, b.hi.z)};
+    return r;
+}
+
+void set_comp(V3& v, int a, float x) { (a == 0 ? v.x : (a == 1 ? v.y : v.z)) = x; }
+
+// Neighbouring floats (finite inputs; +-0 step to the smallest denormal).
+inline float float_up(float f) {
+    const uint32_t u = f2u(f);
+    if ((u & 0x7FFFFFFFu) == 0) return u2f(1u);
+    if (f == INFINITY) return f;
+    return u2f((u & 0x80000000u) ? u - 1 : u + 1);
+}
+inline float float_down(float f) { return -float_up(-f); }
+
+// Grows `b` by a point computed in double, rounded outwards to float and
+// widened by one more ulp, so the box holds the exact point.  The split axis
+// gets `axis_value` exactly.
+void grow_outward(Box& b, const double p[3], int axis, float axis_value) {
+    V3 lo{0.0f, 0.0f, 0.0f}, hi{0.0f, 0.0f, 0.0f};
+    for (int k = 0; k < 3; ++k) {
+        const float f = (float)p[k];
+        set_comp(lo, k, float_down((double)f > p[k] ? float_down(f) : f));
+        set_comp(hi, k, float_up((double)f < p[k] ? float_up(f) : f));
+    }
+    set_comp(lo, axis, axis_value);
+    set_comp(hi, axis, axis_value);
+    b.grow(lo);
+    b.grow(hi);
+}
+
+// Boxes of the parts of triangle (v0, v1, v2) on either side of the plane
+// x[axis] = pos (Stich et al. 2009, "Spatial splits in bounding volume
+// hierarchies"): the vertices on each side plus the points where edges
+// cross the plane.  Both boxes are conservative.
+void split_triangle(const V3 v[3], int axis, float pos, Box& L, Box& R) {
+    L = Box();
+    R = Box();
+    for (int i = 0; i < 3; ++i) {
+        const V3 p = v[i], q = v[(i + 1) % 3];
+        const float pa = comp(p, axis), qa = comp(q, axis);
+        if (pa <= pos) L.grow(p);
+        if (pa >= pos) R.grow(p);
+        if ((pa < pos && qa > pos) || (pa > pos && qa < pos)) {
+            const double t = ((double)pos - pa) / ((double)qa - pa);
+            const double x[3] = {p.x + ((double)q.x - p.x) * t, p.y + ((double)q.y - p.y) * t,
+                                 p.z + ((double)q.z - p.z) * t};
+            grow_outward(L, x, axis, pos);
+            grow_outward(R, x, axis, pos);
+        }
+    }
+}
+
+// The same split for estimates only (spatial binning): float arithmetic, no
+// outward rounding.
+void split_triangle_estimate(const V3 v[3], int axis, float pos, Box& L, Box& R) {
+    L = Box();
+    R = Box();
+    for (int i = 0; i < 3; ++i) {
+        const V3 p = v[i], q = v[(i + 1) % 3];
+        const float pa = comp(p, axis), qa = comp(q, axis);
+        if (pa <= pos) L.grow(p);
+        if (pa >= pos) R.grow(p);
+        if ((pa < pos && qa > pos) || (pa > pos && qa < pos)) {
+            V3 x = p + (q - p) * ((pos - pa) / (qa - pa));
+            set_comp(x, axis, pos);
+            L.grow(x);
+            R.grow(x);
+        }
+    }
+}
+
 struct Builder {
+    const cwbvh_mesh* mesh;
     std::vector<BNode>& nodes;
+    // Node ranges [b, e) own refs[b, c): the tail [e, c) is room for the
+    // references spatial splits add below the node.
     std::vector<PrimRef>& refs;
     std::atomic<int32_t> next_node{0};
     int max_threads;
     int bins;
     int max_depth;
+    // spatial splits are searched where the children of the best object
+    // split overlap by more than this fraction of the root's area (<= 0: never)
+    float split_alpha;
+    double root_area = 0.0;
 
     // Subtrees at least this large are handed to the worker pool; smaller
     // ones are finished by the thread that split them.
     static constexpr int32_t kTaskMin = 4096;
+    // nodes this large bin their references on all threads
+    static constexpr int32_t kParallelBinMin = 1 << 18;
     struct Task {
-        int32_t idx, b, e, depth;
+        int32_t idx, b, e, c, depth;
     };
     std::mutex mu;
     std::condition_variable cv;
     std::vector<Task> tasks;
     int64_t outstanding = 0;
 
-    Builder(std::vector<BNode>& n, std::vector<PrimRef>& r, int threads, int nbins, int maxd)
-        : nodes(n), refs(r), max_threads(threads), bins(nbins), max_depth(maxd) {}
+    Builder(const cwbvh_mesh* m, std::vector<BNode>& n, std::vector<PrimRef>& r, int threads, int nbins, int maxd,
+            float alpha)
+        : mesh(m), nodes(n), refs(r), max_threads(threads), bins(nbins), max_depth(maxd), split_alpha(alpha) {}
 
     int32_t alloc2() { return next_node.fetch_add(2); }
 
     struct Bin {
         Box box;
         int32_t n = 0;
+    };
+    struct SBin {
+        Box box;
+        int32_t enter = 0, exit = 0;
     };
 
     // Centroid bins of one axis over [lo, hi]: nb bins of equal width, in
@@ -113,22 +206,85 @@ struct Builder {
         }
     };
 
-    // Binned SAH over [b, e).  Returns the partition point or -1 for "no good
-    // split" (then the caller splits at the median).
-    int32_t split(int32_t b, int32_t e, const Box& cb, int depth) {
-        int32_t n = e - b;
-        int log2n = 0;
-        while ((1 << log2n) < n) ++log2n;
-        bool force_median = depth + log2n >= max_depth;
-        int best_axis = -1, best_bin = -1;
-        double best_cost = INFINITY;
-        const int nb = std::min(bins, std::max(4, n));
-        const AxisBins ax[3] = {AxisBins(cb.lo.x, cb.hi.x, nb), AxisBins(cb.lo.y, cb.hi.y, nb),
-                                AxisBins(cb.lo.z, cb.hi.z, nb)};
-        if (!force_median) {
-            thread_local std::vector<Bin> bin;
-            bin.assign(3 * (size_t)nb, Bin());
-            auto bin_range = [&](int32_t lo, int32_t hi, std::vector<Bin>& out) {
+    // Planes of the spatial bins of one axis: plane[k], k = 0..nb, with
+    // plane[0] and plane[nb] the node's bounds.  A reference with extent
+    // [lo, hi] on the axis enters at the last bin whose plane is <= lo and
+    // leaves at the last bin whose plane is < hi (not before it enters), so
+    // that the plane k splits it iff lo < plane[k] < hi.
+    struct SpatialAxis {
+        float plane[257];
+        double lo = 0.0, scale = 0.0;
+        int nb = 0;
+        bool ok = false;
+        SpatialAxis(float l, float h, int n) : nb(n) {
+            const double ext = (double)h - (double)l;
+            ok = ext > 0.0 && std::isfinite(ext);
+            lo = l;
+            scale = ok ? (double)n / ext : 0.0;
+            for (int k = 0; k <= n; ++k) plane[k] = (float)((double)l + ext * k / n);
+            plane[n] = h;
+        }
+        int guess(float x) const {
+            const double f = ((double)x - lo) * scale;
+            if (!(f > 0.0)) return 0;
+            return f >= nb - 1 ? nb - 1 : (int)f;
+        }
+        int enter_bin(float x) const {
+            int k = guess(x);
+            while (k > 0 && plane[k] > x) --k;
+            while (k < nb - 1 && plane[k + 1] <= x) ++k;
+            return k;
+        }
+        int exit_bin(float x, int enter) const {
+            int k = guess(x);
+            while (k > 0 && !(plane[k] < x)) --k;
+            while (k < nb - 1 && plane[k + 1] < x) ++k;
+            return std::max(k, enter);
+        }
+    };
+
+    void triangle(int32_t id, V3 v[3]) const { mesh->tri(id, v[0], v[1], v[2]); }
+
+    // Runs bin_range(lo, hi, out) over [b, e) into `bins_out`, on all threads
+    // for large ranges (partial bins merged in thread order).
+    template <class B, class F, class M>
+    void bin_all(int32_t b, int32_t e, size_t count, std::vector<B>& bins_out, F&& bin_range, M&& merge) {
+        const int32_t n = e - b;
+        if (n >= kParallelBinMin && max_threads > 1) {
+            int t = max_threads;
+            std::vector<std::vector<B>> part(t, std::vector<B>(count));
+            std::vector<std::thread> pool;
+            int32_t chunk = (n + t - 1) / t;
+            for (int i = 0; i < t; ++i) {
+                int32_t lo = b + i * chunk, hi = std::min(e, lo + chunk);
+                if (lo >= hi) break;
+                pool.emplace_back([&, lo, hi, i] { bin_range(lo, hi, part[i]); });
+            }
+            for (auto& th : pool) th.join();
+            for (auto& pt : part)
+                for (size_t k = 0; k < count; ++k) merge(bins_out[k], pt[k]);
+        } else {
+            bin_range(b, e, bins_out);
+        }
+    }
+
+    struct Choice {
+        int kind = 0;  // 0: none, 1: object split, 2: spatial split
+        int axis = -1, bin = -1;
+        float pos = 0.0f;  // spatial split: the plane
+        double cost = INFINITY;
+        Box left, right;   // object split: the children's boxes
+    };
+
+    // Binned SAH over the centroids of [b, e).
+    Choice object_split(int32_t b, int32_t e, int nb, const AxisBins ax[3]) {
+        Choice best;
+        const int32_t n = e - b;
+        thread_local std::vector<Bin> bin;
+        bin.assign(3 * (size_t)nb, Bin());
+        bin_all(
+            b, e, 3 * (size_t)nb, bin,
+            [&](int32_t lo, int32_t hi, std::vector<Bin>& out) {
                 for (int32_t i = lo; i < hi; ++i) {
                     const PrimRef& r = refs[i];
                     for (int a = 0; a < 3; ++a) {
@@ -138,59 +294,176 @@ struct Builder {
                         bb.n++;
                     }
                 }
-            };
-            if (n >= (1 << 18) && max_threads > 1) {
-                int t = max_threads;
-                std::vector<std::vector<Bin>> part(t, std::vector<Bin>(3 * nb));
-                std::vector<std::thread> pool;
-                int32_t chunk = (n + t - 1) / t;
-                for (int i = 0; i < t; ++i) {
-                    int32_t lo = b + i * chunk, hi = std::min(e, lo + chunk);
-                    if (lo >= hi) break;
-                    pool.emplace_back([&, lo, hi, i] { bin_range(lo, hi, part[i]); });
-                }
-                for (auto& th : pool) th.join();
-                for (auto& pt : part)
-                    for (int k = 0; k < 3 * nb; ++k) {
-                        bin[k].box.grow(pt[k].box);
-                        bin[k].n += pt[k].n;
-                    }
-            } else {
-                bin_range(b, e, bin);
+            },
+            [](Bin& into, const Bin& from) {
+                into.box.grow(from.box);
+                into.n += from.n;
+            });
+        double right_cost[257];
+        Box right_box[257];
+        for (int a = 0; a < 3; ++a) {
+            if (!ax[a].ok) continue;
+            Box acc;
+            int32_t cnt = 0;
+            for (int k = nb - 1; k >= 1; --k) {
+                acc.grow(bin[a * nb + k].box);
+                cnt += bin[a * nb + k].n;
+                right_cost[k] = cnt ? acc.area() * cnt : 0.0;
+                right_box[k] = acc;
             }
-            double right_cost[257];
-            for (int a = 0; a < 3; ++a) {
-                if (!ax[a].ok) continue;
-                Box acc;
-                int32_t cnt = 0;
-                for (int k = nb - 1; k >= 1; --k) {
-                    acc.grow(bin[a * nb + k].box);
-                    cnt += bin[a * nb + k].n;
-                    right_cost[k] = cnt ? acc.area() * cnt : 0.0;
-                }
-                acc = Box();
-                cnt = 0;
-                for (int k = 0; k < nb - 1; ++k) {
-                    acc.grow(bin[a * nb + k].box);
-                    cnt += bin[a * nb + k].n;
-                    if (cnt == 0 || cnt == n) continue;
-                    const double c = acc.area() * cnt + right_cost[k + 1];
-                    if (c < best_cost) {
-                        best_cost = c;
-                        best_axis = a;
-                        best_bin = k;
-                    }
+            acc = Box();
+            cnt = 0;
+            for (int k = 0; k < nb - 1; ++k) {
+                acc.grow(bin[a * nb + k].box);
+                cnt += bin[a * nb + k].n;
+                if (cnt == 0 || cnt == n) continue;
+                const double c = acc.area() * cnt + right_cost[k + 1];
+                if (c < best.cost) {
+                    best.cost = c;
+                    best.kind = 1;
+                    best.axis = a;
+                    best.bin = k;
+                    best.left = acc;
+                    best.right = right_box[k + 1];
                 }
             }
         }
-        if (best_axis < 0) return -1;
-        const AxisBins& A = ax[best_axis];
-        auto mid = std::partition(refs.begin() + b, refs.begin() + e, [&](const PrimRef& r) {
-            return A.bin(comp(r.cen, best_axis)) <= best_bin;
-        });
-        int32_t m = (int32_t)(mid - refs.begin());
-        if (m == b || m == e) return -1;
-        return m;
+        return best;
+    }
+
+    // Binned spatial-split SAH over the node box `nb_box`: references are
+    // chopped at the bin planes they straddle, each bin grows by the box of
+    // the triangle's part inside it.  Only planes whose split fits the free
+    // room [e, c) are candidates.
+    Choice spatial_split(int32_t b, int32_t e, int32_t c, const Box& nb_box, int nb) {
+        Choice best;
+        const SpatialAxis S[3] = {SpatialAxis(nb_box.lo.x, nb_box.hi.x, nb), SpatialAxis(nb_box.lo.y, nb_box.hi.y, nb),
+                                  SpatialAxis(nb_box.lo.z, nb_box.hi.z, nb)};
+        if (!S[0].ok && !S[1].ok && !S[2].ok) return best;
+        thread_local std::vector<SBin> sbin;
+        sbin.assign(3 * (size_t)nb, SBin());
+        bin_all(
+            b, e, 3 * (size_t)nb, sbin,
+            [&](int32_t lo, int32_t hi, std::vector<SBin>& out) {
+                V3 v[3];
+                for (int32_t i = lo; i < hi; ++i) {
+                    const PrimRef& r = refs[i];
+                    bool fetched = false;
+                    for (int a = 0; a < 3; ++a) {
+                        if (!S[a].ok) continue;
+                        SBin* ob = &out[(size_t)a * nb];
+                        const int k0 = S[a].enter_bin(comp(r.box.lo, a));
+                        const int k1 = S[a].exit_bin(comp(r.box.hi, a), k0);
+                        ob[k0].enter++;
+                        ob[k1].exit++;
+                        if (k0 == k1) {
+                            ob[k0].box.grow(r.box);
+                            continue;
+                        }
+                        if (!fetched) {
+                            triangle(r.id, v);
+                            fetched = true;
+                        }
+                        Box rest = r.box;
+                        for (int k = k0; k < k1 && !box_empty(rest); ++k) {
+                            Box L, R;
+                            split_triangle_estimate(v, a, S[a].plane[k + 1], L, R);
+                            const Box part = box_and(L, rest);
+                            if (!box_empty(part)) ob[k].box.grow(part);
+                            rest = box_and(R, rest);
+                        }
+                        if (!box_empty(rest)) ob[k1].box.grow(rest);
+                    }
+                }
+            },
+            [](SBin& into, const SBin& from) {
+                into.box.grow(from.box);
+                into.enter += from.enter;
+                into.exit += from.exit;
+            });
+        for (int a = 0; a < 3; ++a) {
+            if (!S[a].ok) continue;
+            const SBin* sb = &sbin[(size_t)a * nb];
+            double right_cost[257];
+            int32_t right_n[257];
+            Box acc;
+            int32_t cnt = 0;
+            for (int k = nb - 1; k >= 1; --k) {
+                acc.grow(sb[k].box);
+                cnt += sb[k].exit;
+                right_cost[k] = cnt ? acc.area() * cnt : 0.0;
+                right_n[k] = cnt;
+            }
+            acc = Box();
+            cnt = 0;
+            for (int k = 0; k < nb - 1; ++k) {
+                acc.grow(sb[k].box);
+                cnt += sb[k].enter;
+                if (cnt == 0 || right_n[k + 1] == 0) continue;
+                if ((int64_t)cnt + right_n[k + 1] > (int64_t)c - b) continue;
+                const double cost = acc.area() * cnt + right_cost[k + 1];
+                if (cost < best.cost) {
+                    best.cost = cost;
+                    best.kind = 2;
+                    best.axis = a;
+                    best.bin = k + 1;
+                    best.pos = S[a].plane[k + 1];
+                }
+            }
+        }
+        return best;
+    }
+
+    // Moves the right part [m, e) of an in-place partition so that the left
+    // child gets its share of the free room [e, c), proportional to its size.
+    // Returns the first index of the right child.
+    int32_t place_right(int32_t b, int32_t m, int32_t e, int32_t c) {
+        const int64_t room = (int64_t)c - e;
+        if (room <= 0) return m;
+        const int32_t extra = (int32_t)(room * (m - b) / (e - b));
+        if (extra > 0) std::move_backward(refs.begin() + m, refs.begin() + e, refs.begin() + e + extra);
+        return m + extra;
+    }
+
+    // Spatial partition at plane `pos` of `axis`: references entirely on one
+    // side go there, straddling ones are split into both.  Returns false
+    // (refs untouched) when a side would be empty or the room is short.
+    bool spatial_partition(int32_t b, int32_t e, int32_t c, int axis, float pos, int32_t& m_out,
+                           int32_t& rb_out, int32_t& re_out) {
+        thread_local std::vector<PrimRef> tl, tr;
+        tl.clear();
+        tr.clear();
+        V3 v[3];
+        for (int32_t i = b; i < e; ++i) {
+            const PrimRef& r = refs[i];
+            const float lo = comp(r.box.lo, axis), hi = comp(r.box.hi, axis);
+            if (lo >= pos) {
+                tr.push_back(r);
+            } else if (hi <= pos) {
+                tl.push_back(r);
+            } else {
+                triangle(r.id, v);
+                Box L, R;
+                split_triangle(v, axis, pos, L, R);
+                L = box_and(L, r.box);
+                R = box_and(R, r.box);
+                set_comp(L.hi, axis, std::min(comp(L.hi, axis), pos));
+                set_comp(R.lo, axis, std::max(comp(R.lo, axis), pos));
+                if (!box_empty(L)) tl.push_back(PrimRef{L, L.center(), r.id});
+                if (!box_empty(R)) tr.push_back(PrimRef{R, R.center(), r.id});
+            }
+        }
+        const int64_t nl = (int64_t)tl.size(), nr = (int64_t)tr.size();
+        if (nl == 0 || nr == 0 || nl + nr > (int64_t)c - b) return false;
+        const int64_t room = (int64_t)c - b - nl - nr;
+        const int32_t extra = (int32_t)(room * nl / (nl + nr));
+        std::copy(tl.begin(), tl.end(), refs.begin() + b);
+        const int32_t rb = b + (int32_t)nl + extra;
+        std::copy(tr.begin(), tr.end(), refs.begin() + rb);
+        m_out = b + (int32_t)nl;
+        rb_out = rb;
+        re_out = rb + (int32_t)nr;
+        return true;
     }
 
     void enqueue(const Task& t) {
@@ -212,7 +485,7 @@ struct Builder {
                 t = tasks.back();
                 tasks.pop_back();
             }
-            build(t.idx, t.b, t.e, t.depth);
+            build(t.idx, t.b, t.e, t.c, t.depth);
             bool last;
             {
                 std::lock_guard<std::mutex> g(mu);
@@ -222,20 +495,22 @@ struct Builder {
         }
     }
 
-    // Build the whole tree over refs[0, n) with the worker pool.
+    // Build the whole tree over refs[0, n), room up to refs.size(), with the
+    // worker pool.
     void run(int32_t n) {
         next_node = 1;
+        const int32_t c = (int32_t)refs.size();
         if (max_threads <= 1 || n < 2 * kTaskMin) {
-            build(0, 0, n, 1);
+            build(0, 0, n, c, 1);
             return;
         }
-        enqueue(Task{0, 0, n, 1});
+        enqueue(Task{0, 0, n, c, 1});
         std::vector<std::thread> pool;
         for (int i = 0; i < max_threads; ++i) pool.emplace_back([this] { worker(); });
         for (auto& th : pool) th.join();
     }
 
-    void build(int32_t idx, int32_t b, int32_t e, int depth) {
+    void build(int32_t idx, int32_t b, int32_t e, int32_t c, int depth) {
         BNode& nd = nodes[idx];
         Box box, cb;
         for (int32_t i = b; i < e; ++i) {
@@ -249,7 +524,43 @@ struct Builder {
             nd.count = 1;
             return;
         }
-        int32_t m = split(b, e, cb, depth);
+        const int32_t n = e - b;
+        int log2n = 0;
+        while ((1 << log2n) < n) ++log2n;
+        const bool force_median = depth + log2n >= max_depth;
+        const int nb = std::min(bins, std::max(4, n));
+        const AxisBins ax[3] = {AxisBins(cb.lo.x, cb.hi.x, nb), AxisBins(cb.lo.y, cb.hi.y, nb),
+                                AxisBins(cb.lo.z, cb.hi.z, nb)};
+        int32_t m = -1, rb = -1, re = e;
+        if (!force_median) {
+            Choice obj = object_split(b, e, nb, ax);
+            // overlap of the object split's children, relative to the root
+            bool try_spatial = split_alpha > 0.0f && c > e && root_area > 0.0;
+            if (try_spatial && obj.kind == 1) {
+                const Box ov = box_and(obj.left, obj.right);
+                try_spatial = !box_empty(ov) && ov.area() > split_alpha * root_area;
+            }
+            if (try_spatial) {
+                const Choice sp = spatial_split(b, e, c, box, nb);
+                if (sp.kind == 2 && sp.cost < obj.cost &&
+                    spatial_partition(b, e, c, sp.axis, sp.pos, m, rb, re)) {
+                    obj.kind = 0;
+                }
+            }
+            if (m < 0 && obj.kind == 1) {
+                const AxisBins& A = ax[obj.axis];
+                const int axis = obj.axis, bin = obj.bin;
+                auto mid = std::partition(refs.begin() + b, refs.begin() + e, [&](const PrimRef& r) {
+                    return A.bin(comp(r.cen, axis)) <= bin;
+                });
+                const int32_t mm = (int32_t)(mid - refs.begin());
+                if (mm != b && mm != e) {
+                    m = mm;
+                    rb = place_right(b, m, e, c);
+                    re = rb + (e - m);
+                }
+            }
+        }
         if (m < 0) {
             // median split along the widest centroid axis
             V3 d = cb.hi - cb.lo;
@@ -260,26 +571,32 @@ struct Builder {
                                  const float cp = comp(p.cen, a), cq = comp(q.cen, a);
                                  return cp < cq || (cp == cq && p.id < q.id);
                              });
+            rb = place_right(b, m, e, c);
+            re = rb + (e - m);
         }
-        int32_t c = alloc2();
-        nd.left = c;
-        nd.right = c + 1;
-        // the larger child goes to the pool when it is worth a task; this
-        // thread continues with the other (the tree does not depend on which
-        // thread builds what, only node numbering does)
-        const bool left_big = (m - b) >= (e - m);
-        const int32_t bb = left_big ? b : m, be = left_big ? m : e, sb = left_big ? m : b, se = left_big ? e : m;
-        const int32_t bidx = left_big ? c : c + 1, sidx = left_big ? c + 1 : c;
-        if (max_threads > 1 && be - bb >= kTaskMin) enqueue(Task{bidx, bb, be, depth + 1});
-        else build(bidx, bb, be, depth + 1);
-        build(sidx, sb, se, depth + 1);
+        int32_t ch = alloc2();
+        nd.left = ch;
+        nd.right = ch + 1;
+        // children: left [b, m) with room up to rb, right [rb, re) with room up
+        // to c.  The larger child goes to the pool when it is worth a task;
+        // this thread continues with the other (the tree does not depend on
+        // which thread builds what, only node numbering does).
+        const bool left_big = (m - b) >= (re - rb);
+        const Task L{ch, b, m, rb, depth + 1}, R{ch + 1, rb, re, c, depth + 1};
+        const Task& big = left_big ? L : R;
+        const Task& small = left_big ? R : L;
+        if (max_threads > 1 && big.e - big.b >= kTaskMin) enqueue(big);
+        else build(big.idx, big.b, big.e, big.c, big.depth);
+        build(small.idx, small.b, small.e, small.c, small.depth);
     }
 };
 
 void build_tree(const cwbvh_mesh* mesh, const cwbvh_build_params& P, Tree& T) {
     int64_t nt = mesh->nt();
     int threads = num_threads_or_default(P.num_threads);
-    std::vector<PrimRef> refs((size_t)nt);
+    // room for the references spatial splits add
+    const int64_t cap = P.split_alpha > 0.0f ? nt + (int64_t)((double)nt * P.split_budget) : nt;
+    std::vector<PrimRef> refs((size_t)cap);
     parallel_for(nt, threads, [&](int64_t b, int64_t e) {
         for (int64_t t = b; t < e; ++t) {
             V3 a, bb, c;
@@ -293,7 +610,7 @@ void build_tree(const cwbvh_mesh* mesh, const cwbvh_build_params& P, Tree& T) {
             r.id = (int32_t)t;
         }
     });
-    T.nodes.assign(std::max<int64_t>(1, 2 * nt - 1), BNode());
+    T.nodes.assign(std::max<int64_t>(1, 2 * cap - 1), BNode());
     T.root = 0;
     if (nt == 0) {
         T.nodes.resize(1);
@@ -301,11 +618,17 @@ void build_tree(const cwbvh_mesh* mesh, const cwbvh_build_params& P, Tree& T) {
         T.nodes[0].prims = 0;
         return;
     }
-    Builder B(T.nodes, refs, threads, P.sah_bins, P.max_depth);
+    Builder B(mesh, T.nodes, refs, threads, P.sah_bins, P.max_depth, P.split_alpha);
+    Box root;
+    for (int64_t t = 0; t < nt; ++t) root.grow(refs[(size_t)t].box);
+    B.root_area = root.area();
+    if (!(B.root_area > 0.0) || !std::isfinite(B.root_area)) B.split_alpha = 0.0f;
     B.run((int32_t)nt);
     T.nodes.resize(B.next_node.load());
-    T.refs.resize((size_t)nt);
-    parallel_for(nt, threads, [&](int64_t b, int64_t e) {
+    // leaves name ranges of refs; entries outside every leaf's range are
+    // never read
+    T.refs.resize((size_t)cap);
+    parallel_for(cap, threads, [&](int64_t b, int64_t e) {
         for (int64_t i = b; i < e; ++i) T.refs[(size_t)i] = refs[(size_t)i].id;
     });
 }
@@ -384,6 +707,18 @@ void gather_prims(const Tree& T, int32_t n, std::vector<int32_t>& st, std::vecto
     }
 }
 
+// Drops repeated triangle ids (a triangle split into several references
+// can land twice in one leaf), keeping the first of each.
+void unique_in_order(std::vector<int32_t>& v) {
+    size_t n = 0;
+    for (size_t i = 0; i < v.size(); ++i) {
+        bool seen = false;
+        for (size_t j = 0; j < n && !seen; ++j) seen = v[j] == v[i];
+        if (!seen) v[n++] = v[i];
+    }
+    v.resize(n);
+}
+
 // ------------------------------------------------------------------ BVH2
 
 void layout_bvh2(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params& P, cwbvh_layout& L) {
@@ -424,6 +759,7 @@ void layout_bvh2(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params
     auto gather = [&](int32_t n, std::vector<int32_t>& out) {
         out.clear();
         gather_prims(T, n, scratch, out);
+        unique_in_order(out);
     };
     auto emit_leaf = [&](const std::vector<int32_t>& prims) -> int32_t {
         int32_t addr = (int32_t)L.woop.size();
@@ -743,6 +1079,7 @@ void layout_cwbvh(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_param
             } else {
                 prims.clear();
                 gather_prims(T, bn, scratch, prims);
+                unique_in_order(prims);
                 int cnt = (int)prims.size();
                 unsigned unary = cnt == 1 ? 1u : (cnt == 2 ? 3u : 7u);
                 meta[s] = (uint8_t)((unary << 5) | (unsigned)tri_count);
@@ -769,7 +1106,8 @@ bool valid_params(const cwbvh_build_params& P) {
     return P.sah_bins >= 4 && P.sah_bins <= 256 && P.bvh2_max_leaf >= 1 && P.bvh2_max_leaf <= 32 &&
            (P.cwbvh_node_stride == 5 || P.cwbvh_node_stride == 8) && P.max_depth >= 8 &&
            P.max_depth <= CWBVH_BVH2_STACK_CAPACITY - 2 && P.bvh2_node_cost >= 0 && P.bvh2_tri_cost > 0 &&
-           P.cwbvh_node_cost >= 0 && P.cwbvh_tri_cost > 0;
+           P.cwbvh_node_cost >= 0 && P.cwbvh_tri_cost > 0 && P.split_alpha >= 0.0f && P.split_alpha <= 1.0f &&
+           P.split_budget >= 0.0f && P.split_budget <= 4.0f;
 }
 
 }  // namespace
@@ -793,6 +1131,8 @@ int32_t cwbvh_build_params_default(cwbvh_build_params* p) {
     p->cwbvh_tri_cost = 0.3f;
     p->cwbvh_node_stride = 5;
     p->max_depth = 60;
+    p->split_alpha = 0.0f;
+    p->split_budget = 0.3f;
     return CWBVH_OK;
 }
 

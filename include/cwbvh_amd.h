@@ -198,6 +198,16 @@ typedef struct cwbvh_build_params {
     float cwbvh_tri_cost;       /* collapse DP: cost of a triangle */
     int32_t cwbvh_node_stride;  /* float4 per CWBVH node: 5 (80 B) or 8 (128 B) */
     int32_t max_depth;          /* BVH2 depth limit (<= CWBVH_BVH2_STACK_CAPACITY-2) */
+    /* Spatial splits (SBVH): a node whose best object split leaves children
+     * overlapping by more than split_alpha x the root's surface area also
+     * tries splitting references at bin planes (0 = never, the default; 1e-5
+     * is the usual setting; max 1).  A split triangle sits at several Woop
+     * addresses: a closest hit's t, u, v and triangle (prim_idx) do not
+     * depend on the traversal order, the address reported for it can. */
+    float split_alpha;
+    /* extra triangle references spatial splits may add, as a fraction of the
+     * triangle count (0..4) */
+    float split_budget;
 } cwbvh_build_params;
 
 int32_t cwbvh_build_params_default(cwbvh_build_params* params);

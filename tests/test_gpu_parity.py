@@ -434,6 +434,54 @@ def deep_chain_cwbvh(cw, depth):
     return arr, woop, prim
 
 
+@pytest.fixture(scope="module")
+def split_scene(cw):
+    """Long thin triangles crossing the unit cube among small ones, built
+    with spatial splits (triangles referenced from several leaves)."""
+    rng = np.random.default_rng(31)
+    a = rng.uniform(0, 1, size=(2000, 3))
+    b = rng.uniform(0, 1, size=(2000, 3))
+    c = a + rng.normal(scale=0.01, size=(2000, 3))
+    small = rng.uniform(0, 1, size=(40000, 1, 3)) + rng.normal(scale=0.01, size=(40000, 3, 3))
+    v = np.concatenate([np.stack([a, b, c], axis=1), small]).astype(np.float32).reshape(-1, 3)
+    mesh = cw.Mesh.from_arrays(v, np.arange(len(v), dtype=np.int32).reshape(-1, 3))
+    say("building split scene")
+    bvh2, cwb = cw.build_layouts(mesh, cw.default_build_params(split_alpha=1e-5, split_budget=0.5))
+    assert cwb.stats()["num_tri_refs"] > mesh.num_triangles
+    rays = np.zeros((200_000, 8), np.float32)
+    rays[:, :3] = rng.uniform(-0.2, 1.2, size=(len(rays), 3))
+    d = rng.normal(size=(len(rays), 3))
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 7] = np.inf
+    return {"bvh2": bvh2, "cwbvh": cwb}, rays
+
+
+@pytest.mark.parametrize("kind,kw", [("bvh2", None), ("cwbvh", dict(loop_mode=0)), ("cwbvh", dict(loop_mode=1)),
+                                     ("cwbvh", dict(loop_mode=2)), ("cwbvh", dict(loop_mode=2, postpone_pct=40)),
+                                     ("cwbvh", dict(persistent=0, block_size=64, lds_depth=2))])
+def test_spatial_split_layouts(cw, orc, split_scene, kind, kw):
+    """A split triangle sits at several addresses: which one a closest hit
+    reports follows the traversal order, so the contract is equal t, u, v
+    bits and the same triangle; occlusion agrees exactly."""
+    lays, rays = split_scene
+    lay = lays[kind]
+    sc = cw.Scene(lay)
+    cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw) if kw is not None else None
+    for any_hit in (False, True):
+        say(f"split scene {kind} {kw} any={any_hit} n={len(rays)}")
+        ref = oracle_trace(orc, lay, rays, any_hit)
+        g = sc.trace_numpy(rays, cw.TRACE_ANY if any_hit else cw.TRACE_CLOSEST, cfg)
+        ig, ir = orc.hit_ids(g), orc.hit_ids(ref)
+        assert np.array_equal(ig >= 0, ir >= 0)
+        if any_hit:
+            continue
+        assert np.array_equal(bits(g)[:, [0, 2, 3]], bits(ref)[:, [0, 2, 3]])
+        hit = ig >= 0
+        assert np.array_equal(lay.prim_idx[ig[hit]], lay.prim_idx[ir[hit]])
+        assert 0.05 < hit.mean() < 0.95
+    assert not cw.stack_overflowed()
+
+
 def test_stack_overflow_is_flagged_not_faulted(cw, orc):
     ray = np.array([[-1, -1, -1, 0, *(np.ones(3) / np.sqrt(3)), 1e30]], np.float32)
     # within capacity: traversal equals the oracle

@@ -4,13 +4,6 @@ import numpy as np
 import pytest
 
 
-def _tri_bounds(mesh, prims):
-    v = mesh.vertices.astype(np.float64)
-    idx = mesh.indices
-    pts = v[idx[prims].reshape(-1)]
-    return pts.min(axis=0), pts.max(axis=0)
-
-
 def walk_cwbvh(lay):
     """Yield (node_index, slot, kind, payload) for every non-empty child slot."""
     nodes = lay.nodes
@@ -46,6 +39,28 @@ def walk_cwbvh(lay):
     return out
 
 
+def check_coverage(mesh, leaf_prims, leaf_lo, leaf_hi, samples=6, seed=0):
+    """Every point of every triangle lies in the box of some leaf that holds
+    the triangle, every box on that leaf's root path included (leaf_lo/hi
+    are those intersections).  Points: the vertices and random interior
+    points.  With spatial splits a triangle's leaves hold only parts of it,
+    so this is the conservativeness that matters for traversal."""
+    v = mesh.vertices.astype(np.float64)[mesh.indices]                 # (nt, 3, 3)
+    rng = np.random.default_rng(seed)
+    w = rng.dirichlet([1.0, 1.0, 1.0], size=samples)
+    pts = np.concatenate([v, np.einsum("kj,tjc->tkc", w, v)], axis=1)  # (nt, 3 + samples, 3)
+    covered = np.zeros(pts.shape[:2], bool)
+    prim = np.asarray(leaf_prims, np.int64)
+    lo = np.asarray(leaf_lo, np.float64)[:, None, :]
+    hi = np.asarray(leaf_hi, np.float64)[:, None, :]
+    q = pts[prim]
+    tol = 1e-12 * np.abs(q)  # the rounding of the float64 interior points
+    inside = np.all((q + tol >= lo) & (q - tol <= hi), axis=2)
+    np.logical_or.at(covered, prim, inside)
+    bad = np.argwhere(~covered)
+    assert len(bad) == 0, f"uncovered triangle points {bad[:5]}"
+
+
 def check_cwbvh(mesh, lay, orc):
     nodes = lay.nodes
     S = lay.node_stride
@@ -54,42 +69,52 @@ def check_cwbvh(mesh, lay, orc):
     assert len(refs) == len(set(refs)) == lay.stats()["num_tri_refs"]
     prims = lay.prim_idx[np.array(refs, np.int64)]
     assert np.all(prims >= 0)
-    assert sorted(prims.tolist()) == list(range(mesh.num_triangles))
+    # every triangle is referenced, at most once per leaf (spatial splits may
+    # reference it from several leaves)
+    assert set(prims.tolist()) == set(range(mesh.num_triangles))
+    for (_, _, kind, p, _) in slots:
+        if kind == "leaf":
+            ids = lay.prim_idx[np.array(p)].tolist()
+            assert len(ids) == len(set(ids))
     # every node except the root is the child of exactly one inner slot
     inner_children = [p for (_, _, kind, p, _) in slots if kind == "inner"]
     assert sorted(inner_children) == list(range(1, len(nodes) // S))
-    # conservativeness: decoded child box contains the child's triangles
+    for (_, _, _, _, e) in slots:
+        assert np.all((e >= -126) & (e <= 127))
+    # conservativeness: the decoded boxes on the path to a leaf contain the
+    # parts of its triangles that the leaf stands for
     by_node = {}
     for ni, s, kind, p, e in slots:
         by_node.setdefault(ni, []).append((s, kind, p))
-        assert np.all((e >= -126) & (e <= 127))
-    memo = {}
-
-    def subtree_prims(ni):
-        if ni in memo:
-            return memo[ni]
-        acc = []
-        for s, kind, p in by_node.get(ni, []):
-            acc += subtree_prims(p) if kind == "inner" else list(lay.prim_idx[np.array(p)])
-        memo[ni] = acc
-        return acc
-
-    for ni, lst in by_node.items():
+    eff = {0: (np.full(3, -np.inf), np.full(3, np.inf))}
+    order = [0]
+    leaf_prims, leaf_lo, leaf_hi = [], [], []
+    while order:
+        ni = order.pop()
+        plo, phi = eff[ni]
         lo, hi = orc.cwbvh_decode(nodes[ni * S:ni * S + 5])
-        for s, kind, p in lst:
-            pr = subtree_prims(p) if kind == "inner" else list(lay.prim_idx[np.array(p)])
-            blo, bhi = _tri_bounds(mesh, np.array(pr))
-            assert np.all(lo[s] <= blo) and np.all(hi[s] >= bhi), (ni, s)
+        for s, kind, p in by_node.get(ni, []):
+            blo, bhi = np.maximum(plo, lo[s]), np.minimum(phi, hi[s])
+            if kind == "inner":
+                eff[p] = (blo, bhi)
+                order.append(p)
+            else:
+                for pr in lay.prim_idx[np.array(p)]:
+                    leaf_prims.append(pr)
+                    leaf_lo.append(blo)
+                    leaf_hi.append(bhi)
+    check_coverage(mesh, leaf_prims, leaf_lo, leaf_hi)
 
 
 def check_bvh2(mesh, lay):
     nodes = lay.nodes
     woop = lay.woop
     seen = []
-    stack = [0]
+    stack = [(0, np.full(3, -np.inf), np.full(3, np.inf))]
     visited = 0
+    leaf_prims, leaf_lo, leaf_hi = [], [], []
     while stack:
-        a = stack.pop()
+        a, plo, phi = stack.pop()
         assert a % 4 == 0 and a + 3 < len(nodes)
         visited += 1
         n = nodes[a:a + 4]
@@ -97,8 +122,9 @@ def check_bvh2(mesh, lay):
                  (np.array([n[1, 0], n[1, 2], n[2, 2]]), np.array([n[1, 1], n[1, 3], n[2, 3]]))]
         ch = n[3].view(np.int32)[:2]
         for c, (blo, bhi) in zip(ch, boxes):
+            elo, ehi = np.maximum(plo, blo), np.minimum(phi, bhi)
             if c >= 0:
-                stack.append(int(c))
+                stack.append((int(c), elo, ehi))
                 continue
             w = ~int(c)
             prims = []
@@ -107,12 +133,16 @@ def check_bvh2(mesh, lay):
                 assert np.all(lay.prim_idx[w + 1:w + 3] == -1)
                 w += 3
             assert lay.prim_idx[w] == -1
-            if prims:
-                tlo, thi = _tri_bounds(mesh, np.array(prims))
-                assert np.all(blo <= tlo) and np.all(bhi >= thi)
+            assert len(prims) == len(set(prims))
+            for pr in prims:
+                leaf_prims.append(pr)
+                leaf_lo.append(elo)
+                leaf_hi.append(ehi)
             seen += prims
     assert visited == lay.stats()["num_nodes"]
-    assert sorted(seen) == list(range(mesh.num_triangles))
+    assert len(seen) == lay.stats()["num_tri_refs"]
+    assert set(seen) == set(range(mesh.num_triangles))
+    check_coverage(mesh, leaf_prims, leaf_lo, leaf_hi)
 
 
 @pytest.mark.parametrize("which", ["heightfield", "soup", "planes", "spheres"])
@@ -185,14 +215,17 @@ def test_depth_limit_respected(cw):
     assert cwb.stats()["stack_need"] <= 21
 
 
-def test_build_same_for_any_thread_count(cw):
+@pytest.mark.parametrize("split_alpha", [0.0, 1e-5])
+def test_build_same_for_any_thread_count(cw, split_alpha):
     """The parallel builder (work queue over subtrees, parallel binning of
     large nodes, Woop rows in parallel, both layouts side by side) returns
-    the same bytes as the single-threaded one."""
+    the same bytes as the single-threaded one, spatial splits included."""
     mesh = cw.Mesh.cluster_soup(300_000, 16, 0.002, 0.2, -5.0, 0.5, seed=8)
-    ref = cw.build_layouts(mesh, cw.default_build_params(num_threads=1))
+    ref = cw.build_layouts(mesh, cw.default_build_params(num_threads=1, split_alpha=split_alpha))
+    if split_alpha > 0:
+        assert ref[1].stats()["num_tri_refs"] > mesh.num_triangles
     for t in (3, 8):
-        got = cw.build_layouts(mesh, cw.default_build_params(num_threads=t))
+        got = cw.build_layouts(mesh, cw.default_build_params(num_threads=t, split_alpha=split_alpha))
         for a, b in zip(ref, got):
             assert np.array_equal(a.nodes.view(np.uint32), b.nodes.view(np.uint32))
             assert np.array_equal(a.woop.view(np.uint32), b.woop.view(np.uint32))
@@ -295,3 +328,81 @@ def test_from_arrays_refuses_malformed_layouts(cw):
     # the untouched arrays are accepted
     assert cw.Layout.from_arrays(cw.KIND_CWBVH, 5, cwb.nodes, cwb.woop, cwb.prim_idx).stats()["num_nodes"] == \
         cwb.stats()["num_nodes"]
+
+
+def assert_same_hits(a, b, prim_idx, orc, prim_idx_b=None):
+    """Equal t, u, v bits and the same triangle behind the two hit addresses."""
+    prim_idx_b = prim_idx if prim_idx_b is None else prim_idx_b
+    assert np.array_equal(a[:, [0, 2, 3]].view(np.uint32), b[:, [0, 2, 3]].view(np.uint32))
+    ia, ib = orc.hit_ids(a), orc.hit_ids(b)
+    assert np.array_equal(ia >= 0, ib >= 0)
+    hit = ia >= 0
+    assert np.array_equal(prim_idx[ia[hit]], prim_idx_b[ib[hit]])
+
+
+def _slivers(cw, n_long, n_small, seed):
+    """Long thin triangles crossing the unit cube among small ones: the case
+    spatial splits are for (object splits leave heavily overlapping boxes)."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0, 1, size=(n_long, 3))
+    b = rng.uniform(0, 1, size=(n_long, 3))
+    c = a + rng.normal(scale=0.01, size=(n_long, 3))
+    cen = rng.uniform(0, 1, size=(n_small, 1, 3))
+    small = cen + rng.normal(scale=0.01, size=(n_small, 3, 3))
+    v = np.concatenate([np.stack([a, b, c], axis=1), small]).astype(np.float32).reshape(-1, 3)
+    return cw.Mesh.from_arrays(v, np.arange(len(v), dtype=np.int32).reshape(-1, 3))
+
+
+@pytest.mark.parametrize("stride", [5, 8])
+def test_spatial_splits(cw, orc, stride):
+    """Spatial splits reference a triangle from several leaves, each with the
+    box of its part: layouts stay conservative (every triangle point is in
+    a leaf box that holds it), traversal equals brute force bit for bit, and
+    the per-ray work drops against the object-split build."""
+    mesh = _slivers(cw, 400, 4000, seed=12)
+    nt = mesh.num_triangles
+    off = cw.build_layouts(mesh, cw.default_build_params(split_alpha=0.0, cwbvh_node_stride=stride))
+    on = cw.build_layouts(mesh, cw.default_build_params(split_alpha=1e-5, split_budget=0.5,
+                                                        cwbvh_node_stride=stride))
+    for lay in off:
+        assert lay.stats()["num_tri_refs"] == nt
+    bvh2, cwb = on
+    assert nt < bvh2.stats()["num_tri_refs"] <= nt + int(0.5 * nt)
+    assert nt < cwb.stats()["num_tri_refs"] <= nt + int(0.5 * nt)
+    check_cwbvh(mesh, cwb, orc)
+    check_bvh2(mesh, bvh2)
+    rng = np.random.default_rng(13)
+    rays = np.zeros((3000, 8), np.float32)
+    rays[:, :3] = rng.uniform(-0.2, 1.2, size=(3000, 3))
+    d = rng.normal(size=(3000, 3))
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 7] = np.inf
+    rows = np.arange(0, len(cwb.woop), 3, dtype=np.int32)[cwb.prim_idx[::3] >= 0]
+    ref = orc.trace_brute(cwb.woop, rows, rays)
+    got, k_on = orc.trace_cwbvh(cwb.nodes, stride, cwb.woop, rays, counters=True)
+    # a split triangle has several addresses: brute force reports the lowest,
+    # traversal the first copy it tests; t, u, v and the triangle agree
+    assert_same_hits(got, ref, cwb.prim_idx, orc)
+    for any_hit in (False, True):
+        h2 = orc.trace_bvh2(bvh2.nodes, bvh2.woop, rays, any_hit=any_hit)
+        h8 = orc.trace_cwbvh(cwb.nodes, stride, cwb.woop, rays, any_hit=any_hit)
+        assert np.array_equal(orc.hit_ids(h2) >= 0, orc.hit_ids(h8) >= 0)
+    # the same nearest triangle and distance as the object-split layout
+    base = orc.trace_cwbvh(off[1].nodes, stride, off[1].woop, rays)
+    assert_same_hits(got, base, cwb.prim_idx, orc, off[1].prim_idx)
+    _, k_off = orc.trace_cwbvh(off[1].nodes, stride, off[1].woop, rays, counters=True)
+    assert k_on[0] + k_on[1] < k_off[0] + k_off[1]
+
+
+def test_spatial_split_budget_and_params(cw):
+    mesh = _slivers(cw, 300, 2000, seed=14)
+    nt = mesh.num_triangles
+    assert cw.default_build_params().split_alpha == 0.0
+    for budget in (0.0, 0.02, 0.2):
+        bvh2, cwb = cw.build_layouts(mesh, cw.default_build_params(split_alpha=1e-5, split_budget=budget))
+        assert nt <= cwb.stats()["num_tri_refs"] <= nt + int(budget * nt)
+        assert nt <= bvh2.stats()["num_tri_refs"] <= nt + int(budget * nt)
+    for bad in (dict(split_alpha=-1.0), dict(split_alpha=2.0), dict(split_budget=-0.1), dict(split_budget=5.0),
+                dict(split_alpha=float("nan"))):
+        with pytest.raises(cw.CwbvhError):
+            cw.build_layouts(mesh, cw.default_build_params(**bad))
