@@ -465,7 +465,7 @@ def main() -> int:
     ap.add_argument("--c3-rays", type=int, default=64 * 1024 * 1024)
     ap.add_argument("--c4-scenes", type=int, default=32)
     ap.add_argument("--c4-tris", type=int, default=100_000_000, help="total triangles of the C4 scene set")
-    ap.add_argument("--c4-requests", type=int, default=400)
+    ap.add_argument("--c4-requests", type=int, default=2000)
     ap.add_argument("--c4-rates", default="1000,2000,5000,10000,20000,50000")
     ap.add_argument("--c2-scenes", type=int, default=256)
     ap.add_argument("--tune-reps", type=int, default=3)

@@ -195,7 +195,7 @@ typedef struct cwbvh_build_params {
     float bvh2_tri_cost;        /* SAH cost of a triangle test (BVH2 leaves) */
     int32_t bvh2_max_leaf;      /* max triangles per BVH2 leaf (1..32) */
     float cwbvh_node_cost;      /* collapse DP: cost of an 8-wide node */
-    float cwbvh_tri_cost;       /* collapse DP: cost of a triangle */
+    float cwbvh_tri_cost;       /* collapse DP: cost of a triangle (default 0.6) */
     int32_t cwbvh_node_stride;  /* float4 per CWBVH node: 5 (80 B) or 8 (128 B) */
     int32_t max_depth;          /* BVH2 depth limit (<= CWBVH_BVH2_STACK_CAPACITY-2) */
     /* Spatial splits (SBVH): a node whose best object split leaves children
