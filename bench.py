@@ -196,8 +196,12 @@ def bench_c3(args, cw, W, dev) -> int:
     n = len(rays)
     log(f"C3: {mesh.num_triangles} tris, {n} rays, gen {gen_s:.1f} s, build {build_s:.1f} s, "
         f"cwbvh {cwb.stats()}, bvh2 {bvh2.stats()}")
+    # the library's default upload (a scene this large gets 128 B nodes: one
+    # cache line per node), plus the 80 B reference layout for the table below
     s_cw, s_b2 = cw.Scene(cwb), cw.Scene(bvh2)
-    s_cw8 = cw.Scene(cwb.restride(8))  # 128 B re-lay: one cache line per node
+    s_cw80 = s_cw if s_cw.node_stride == 5 else cw.Scene(cwb, stride_mode=cw.SCENE_STRIDE_KEEP)
+    s_cw8 = s_cw if s_cw.node_stride == 8 else cw.Scene(cwb, stride_mode=cw.SCENE_STRIDE_128)
+    log(f"default upload: {s_cw.node_stride * 16} B nodes")
     lo, hi = s_cw.bounds()
     del bvh2
     dr = cw.DeviceBuffer.from_numpy(rays)
@@ -227,7 +231,7 @@ def bench_c3(args, cw, W, dev) -> int:
     # node layout x inner-loop schedule: this scene is far larger than L2 and
     # the Infinity Cache, so line-aligned nodes and access order matter
     layouts = []
-    for lname, sc in (("node80", s_cw), ("node128", s_cw8)):
+    for lname, sc in (("node80", s_cw80), ("node128", s_cw8)):
         for mode in (0, 1, 2):
             cfg = cw.default_trace_config(cw.KIND_CWBVH, loop_mode=mode)
             for oname, order in (("shuffled", None), ("morton", do.ptr)):
@@ -259,7 +263,8 @@ def bench_c3(args, cw, W, dev) -> int:
         "data": "synthetic",
         "config": {"workload": f"C3: clustered soup {mesh.num_triangles} triangles (4096 clusters), {n} diffuse "
                                f"rays from surface points, tmax=inf, closest hit, shuffled order",
-                   "triangles": mesh.num_triangles, "rays": n, "seed": W.SEEDS["C3"]},
+                   "triangles": mesh.num_triangles, "rays": n, "seed": W.SEEDS["C3"],
+                   "node_bytes": 16 * s_cw.node_stride},
         "results": res,
         "cwbvh_layout_schedule": layouts,
         "best_cwbvh_shuffled": min((r for r in layouts if r["order"] == "shuffled"), key=lambda r: r["ms"]),

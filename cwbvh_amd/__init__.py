@@ -30,6 +30,9 @@ TRACE_PRIM_ID = 2
 TRACE_STATS = 4
 STACK_CAPACITY = 64
 BVH2_STACK_CAPACITY = 64
+SCENE_STRIDE_AUTO = 0
+SCENE_STRIDE_KEEP = 1
+SCENE_STRIDE_128 = 2
 
 RAY_DTYPE = np.float32  # rays: (n, 8) float32 = origin.xyz, tmin, dir.xyz, tmax
 HIT_DTYPE = np.float32  # hits: (n, 4) float32 = t, triAddr bits, u, v
@@ -225,6 +228,8 @@ _SIGS = {
     "cwbvh_event_elapsed_ms": (_I32, [_P, _P, C.POINTER(_F)]),
     "cwbvh_copy_u4": (_I32, [_P, _P, _I64, _P]),
     "cwbvh_scene_create": (_I32, [_P, _P, _PP]),
+    "cwbvh_scene_create_ex": (_I32, [_P, _P, C.c_int32, _PP]),
+    "cwbvh_scene_node_stride": (_I32, [_P, C.POINTER(C.c_int32)]),
     "cwbvh_scene_wrap_device": (_I32, [_I32, _I32, _P, _P, _P, _I32, _PP]),
     "cwbvh_scene_device_bytes": (_I32, [_P, C.POINTER(_I64)]),
     "cwbvh_scene_kind": (_I32, [_P, C.POINTER(_I32)]),
@@ -735,9 +740,11 @@ def default_trace_config(kind: int, **overrides) -> TraceConfig:
 class Scene:
     """A BVH resident in device memory (owning upload of a host Layout)."""
 
-    def __init__(self, layout: Layout, stream: Stream | None = None):
+    def __init__(self, layout: Layout, stream: Stream | None = None, stride_mode: int = 0):
+        """stride_mode: SCENE_STRIDE_AUTO (128 B nodes for CWBVH scenes above
+        256 MiB), SCENE_STRIDE_KEEP or SCENE_STRIDE_128."""
         s = C.c_void_p()
-        check(lib.cwbvh_scene_create(layout.handle, stream.handle if stream else None, C.byref(s)))
+        check(lib.cwbvh_scene_create_ex(layout.handle, stream.handle if stream else None, stride_mode, C.byref(s)))
         self._s = s
         self.kind = layout.kind
         self._keep = ()
@@ -758,6 +765,13 @@ class Scene:
     @property
     def handle(self) -> C.c_void_p:
         return self._s
+
+    @property
+    def node_stride(self) -> int:
+        """float4 per node of the device node array."""
+        v = C.c_int32()
+        check(lib.cwbvh_scene_node_stride(self._s, C.byref(v)))
+        return v.value
 
     def device_bytes(self) -> int:
         b = C.c_int64()

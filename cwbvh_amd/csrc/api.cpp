@@ -251,23 +251,49 @@ static void layout_bounds(const cwbvh_layout& L, float lo[3], float hi[3]) {
 }
 
 int32_t cwbvh_scene_create(const cwbvh_layout* L, void* stream, cwbvh_scene** out) {
+    return cwbvh_scene_create_ex(L, stream, CWBVH_SCENE_STRIDE_AUTO, out);
+}
+
+int32_t cwbvh_scene_create_ex(const cwbvh_layout* L, void* stream, int32_t stride_mode, cwbvh_scene** out) {
     CW_CHECK_ARG(L != nullptr && out != nullptr, "NULL argument");
     CW_CHECK_ARG(!L->nodes.empty(), "empty node array");
+    CW_CHECK_ARG(stride_mode >= CWBVH_SCENE_STRIDE_AUTO && stride_mode <= CWBVH_SCENE_STRIDE_128,
+                 "stride_mode must be CWBVH_SCENE_STRIDE_AUTO, _KEEP or _128");
     const int cap = L->kind == CWBVH_KIND_CWBVH ? CWBVH_STACK_CAPACITY : CWBVH_BVH2_STACK_CAPACITY;
     if (L->stack_need > cap) {
         set_error("tree needs " + std::to_string(L->stack_need) + " stack entries, kernel holds " +
                   std::to_string(cap));
         return CWBVH_ERR_STACK_DEPTH;
     }
+    // 80 B nodes re-laid at 128 B (same bytes, 48 B of zeros behind each)
+    const cwbvh_float4* node_src = L->nodes.data();
+    size_t node_f4 = L->nodes.size();
+    int32_t stride = L->node_stride;
+    std::vector<cwbvh_float4> relaid;
+    if (L->kind == CWBVH_KIND_CWBVH && L->node_stride == 5) {
+        const int64_t streamed = (int64_t)(L->nodes.size() + L->woop.size()) * (int64_t)sizeof(cwbvh_float4);
+        if (stride_mode == CWBVH_SCENE_STRIDE_128 ||
+            (stride_mode == CWBVH_SCENE_STRIDE_AUTO && streamed > kCacheResidentBytes)) {
+            const int64_t nn = (int64_t)L->nodes.size() / 5;
+            relaid.assign((size_t)nn * 8, cwbvh_float4{0.f, 0.f, 0.f, 0.f});
+            parallel_for(nn, num_threads_or_default(0), [&](int64_t b, int64_t e) {
+                for (int64_t i = b; i < e; ++i)
+                    std::memcpy(&relaid[(size_t)i * 8], &L->nodes[(size_t)i * 5], 5 * sizeof(cwbvh_float4));
+            });
+            node_src = relaid.data();
+            node_f4 = relaid.size();
+            stride = 8;
+        }
+    }
     auto* s = new cwbvh_scene();
     s->kind = L->kind;
-    s->node_stride = L->node_stride;
+    s->node_stride = stride;
     s->stack_need = L->stack_need;
     s->owning = true;
     layout_bounds(*L, s->lo, s->hi);
     s->woop_f4 = (int64_t)L->woop.size();
     hipGetDevice(&s->device);
-    const size_t nb = L->nodes.size() * sizeof(cwbvh_float4);
+    const size_t nb = node_f4 * sizeof(cwbvh_float4);
     // two float4 of zero padding after the Woop rows keep every triangle-sized
     // read of the last leaf inside the allocation
     const size_t wb = (L->woop.size() + 2) * sizeof(cwbvh_float4);
@@ -277,7 +303,7 @@ int32_t cwbvh_scene_create(const cwbvh_layout* L, void* stream, cwbvh_scene** ou
     if (e == hipSuccess) e = hipMalloc(&dw, wb);
     if (e == hipSuccess) e = hipMalloc(&dp, pb);
     hipStream_t st = (hipStream_t)stream;
-    if (e == hipSuccess) e = hipMemcpyAsync(dn, L->nodes.data(), nb, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dn, node_src, nb, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(dw, 0, wb, st);
     if (e == hipSuccess && !L->woop.empty())
         e = hipMemcpyAsync(dw, L->woop.data(), L->woop.size() * sizeof(cwbvh_float4), hipMemcpyHostToDevice, st);
@@ -344,6 +370,12 @@ int32_t cwbvh_scene_device_bytes(const cwbvh_scene* s, int64_t* bytes) {
 int32_t cwbvh_scene_kind(const cwbvh_scene* s, int32_t* kind) {
     CW_CHECK_ARG(s != nullptr && kind != nullptr, "NULL argument");
     *kind = s->kind;
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_scene_node_stride(const cwbvh_scene* s, int32_t* stride) {
+    CW_CHECK_ARG(s != nullptr && stride != nullptr, "NULL argument");
+    *stride = s->node_stride;
     return CWBVH_OK;
 }
 

@@ -293,8 +293,23 @@ int32_t cwbvh_copy_u4(void* dst, const void* src, int64_t n_u4, void* stream);
 
 typedef struct cwbvh_scene cwbvh_scene;
 
-/* Upload a host layout (owning device copy). */
+/* Node layout of an uploaded CWBVH scene (cwbvh_scene_create_ex). */
+#define CWBVH_SCENE_STRIDE_AUTO 0 /* the layout's stride, but a scene of 80 B nodes whose node
+                                   * and Woop arrays exceed 256 MiB gets 128 B nodes */
+#define CWBVH_SCENE_STRIDE_KEEP 1 /* the layout's stride */
+#define CWBVH_SCENE_STRIDE_128 2  /* 128 B nodes */
+
+/* Upload a host layout (owning device copy), CWBVH_SCENE_STRIDE_AUTO. */
 int32_t cwbvh_scene_create(const cwbvh_layout* layout, void* stream, cwbvh_scene** out);
+/* The same with a chosen node layout.  A 128 B node holds the 80 B node's
+ * bytes and 48 B of zeros, so hits are bit-identical for every choice; a
+ * scene that streams from HBM (larger than the 256 MB Infinity Cache) reads
+ * one cache line per node instead of two for most nodes.  BVH2 layouts
+ * ignore stride_mode. */
+int32_t cwbvh_scene_create_ex(const cwbvh_layout* layout, void* stream, int32_t stride_mode,
+                              cwbvh_scene** out);
+/* float4 per node of the scene's device node array (BVH2 4, CWBVH 5 or 8). */
+int32_t cwbvh_scene_node_stride(const cwbvh_scene* scene, int32_t* stride);
 /* Wrap device arrays without taking ownership (bind semantics). prim_idx may
  * be NULL; stack_need = traversal stack entries the tree needs (0 = unknown). */
 int32_t cwbvh_scene_wrap_device(int32_t kind, int32_t node_stride, const void* d_nodes,

@@ -263,6 +263,45 @@ def test_wrapped_scene_and_scene_default_config(cw, c0, c0_ref):
     assert big.default_config().loop_mode == 0
 
 
+def test_scene_node_stride_modes(cw, c0, c0_ref):
+    """Uploads with 128 B nodes trace bit-identically to the 80 B layout; the
+    automatic mode keeps cache-resident scenes at 80 B."""
+    wl, lays = c0
+    for mode, want in ((cw.SCENE_STRIDE_AUTO, 5), (cw.SCENE_STRIDE_KEEP, 5), (cw.SCENE_STRIDE_128, 8)):
+        say(f"stride mode {mode}")
+        sc = cw.Scene(lays["cwbvh"], stride_mode=mode)
+        assert sc.node_stride == want
+        assert sc.device_bytes() > 16 * want * lays["cwbvh"].stats()["num_nodes"]
+        for any_hit in (False, True):
+            assert_bitwise(sc.trace_numpy(wl.rays, int(any_hit)), c0_ref[("cwbvh", any_hit)],
+                           f"stride mode {mode} any={any_hit}")
+    assert cw.Scene(lays["bvh2"], stride_mode=cw.SCENE_STRIDE_128).node_stride == 4
+    with pytest.raises(cw.CwbvhError):
+        cw.Scene(lays["cwbvh"], stride_mode=3)
+
+
+def test_large_scene_uploads_128b_nodes(cw):
+    """A CWBVH scene larger than the Infinity Cache (here 5 M triangles,
+    about 320 MB of nodes and Woop rows) gets 128 B nodes by default, with
+    the hits of the 80 B layout."""
+    say("building 5 M-triangle soup")
+    mesh = cw.Mesh.cluster_soup(5_000_000, 1024, 0.005, 0.05, float(np.log(1e-3)), 0.5, seed=41)
+    _, cwb = cw.build_layouts(mesh, bvh2=False)
+    st = cwb.stats()
+    assert 16 * (st["nodes_f4"] + st["woop_f4"]) > (256 << 20)
+    auto = cw.Scene(cwb)
+    keep = cw.Scene(cwb, stride_mode=cw.SCENE_STRIDE_KEEP)
+    assert auto.node_stride == 8 and keep.node_stride == 5
+    diag = mesh.diagonal()
+    rays = cw.rays_sphere_from_surface(mesh, 200_000, 1e-4 * diag, 0.0, np.inf, seed=42)
+    for any_hit in (False, True):
+        say(f"large scene any={any_hit}")
+        a = auto.trace_numpy(rays, int(any_hit))
+        b = keep.trace_numpy(rays, int(any_hit))
+        assert_bitwise(a, b, f"128 B vs 80 B nodes any={any_hit}")
+    assert auto.default_config().loop_mode == 0
+
+
 def test_invalid_loop_configs_rejected(cw, c0):
     wl, lays = c0
     sc = cw.Scene(lays["cwbvh"])
