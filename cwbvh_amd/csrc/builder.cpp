@@ -663,31 +663,45 @@ void woopify(V3 v0, V3 v1, V3 v2, cwbvh_float4 out[3]) {
     if (f2u(out[0].x) == 0x80000000u) out[0].x = 0.0f;
 }
 
-// A triangle whose three Woop rows start at woop[addr].  The layout passes
-// only reserve the rows; fill_woop computes all of them in parallel once the
-// structure is known.
-struct TriSlot {
-    int64_t addr;
-    int32_t prim;
+// Fork-join over the subtrees of a tree: run() starts a task on a new thread
+// while fewer than `limit` threads work, and runs it on the caller otherwise.
+// The layout tasks write disjoint ranges that are fixed before they start, so
+// the bytes do not depend on which thread ran what.
+class Forker {
+public:
+    explicit Forker(int limit) : limit_(limit) {}
+    template <class F>
+    void run(bool big, std::vector<std::thread>& group, F fn) {
+        if (big && limit_ > 1 && extra_.fetch_add(1) < limit_ - 1) {
+            group.emplace_back([this, fn]() {
+                fn();
+                extra_.fetch_sub(1);
+            });
+            return;
+        }
+        if (big && limit_ > 1) extra_.fetch_sub(1);
+        fn();
+    }
+    static void join(std::vector<std::thread>& group) {
+        for (auto& t : group) t.join();
+        group.clear();
+    }
+    // subtrees with at least this many references are worth a thread
+    static constexpr int32_t kMinPrims = 1 << 14;
+
+private:
+    std::atomic<int> extra_{0};  // threads started beyond the caller's
+    int limit_;
 };
 
-void reserve_tri(int32_t prim, cwbvh_layout& L, std::vector<TriSlot>& slots) {
-    slots.push_back(TriSlot{(int64_t)L.woop.size(), prim});
-    L.woop.resize(L.woop.size() + 3);
-    L.prim_idx.push_back(prim);
-    L.prim_idx.push_back(-1);
-    L.prim_idx.push_back(-1);
-    L.num_tri_refs++;
-}
-
-void fill_woop(const cwbvh_mesh* mesh, const std::vector<TriSlot>& slots, int threads, cwbvh_layout& L) {
-    parallel_for((int64_t)slots.size(), threads, [&](int64_t b, int64_t e) {
-        for (int64_t i = b; i < e; ++i) {
-            V3 a, bb, c;
-            mesh->tri(slots[(size_t)i].prim, a, bb, c);
-            woopify(a, bb, c, &L.woop[(size_t)slots[(size_t)i].addr]);
-        }
-    });
+// Woop rows and prim_idx entries [prim, -1, -1] of one triangle at row `row`.
+void write_triangle(const cwbvh_mesh* mesh, int32_t prim, int64_t row, cwbvh_layout& L) {
+    V3 a, b, c;
+    mesh->tri(prim, a, b, c);
+    woopify(a, b, c, &L.woop[(size_t)row]);
+    L.prim_idx[(size_t)row] = prim;
+    L.prim_idx[(size_t)row + 1] = -1;
+    L.prim_idx[(size_t)row + 2] = -1;
 }
 
 // Triangle ids under tree node n, left to right; `st` is scratch.
@@ -721,28 +735,125 @@ void unique_in_order(std::vector<int32_t>& v) {
 
 // ------------------------------------------------------------------ BVH2
 
+// Child boxes of a BVH2 node at float4 `addr`: (lo.x, hi.x, lo.y, hi.y) per
+// child in rows 0/1, the z extents of both in row 2, child links in row 3.
+void bvh2_put_box(cwbvh_layout& L, int64_t addr, int slot, const Box& b) {
+    cwbvh_float4* n = &L.nodes[(size_t)addr];
+    if (slot == 0) {
+        n[0] = cwbvh_float4{b.lo.x, b.hi.x, b.lo.y, b.hi.y};
+        n[2].x = b.lo.z;
+        n[2].y = b.hi.z;
+    } else {
+        n[1] = cwbvh_float4{b.lo.x, b.hi.x, b.lo.y, b.hi.y};
+        n[2].z = b.lo.z;
+        n[2].w = b.hi.z;
+    }
+}
+
+void bvh2_put_child(cwbvh_layout& L, int64_t addr, int slot, int32_t value) {
+    if (slot == 0) L.nodes[(size_t)addr + 3].x = i2f(value);
+    else L.nodes[(size_t)addr + 3].y = i2f(value);
+}
+
+// The BVH2 layout in the order of a pre-order DFS (left child first): inner
+// nodes take consecutive 64 B slots, each leaf its triangles' Woop rows plus a
+// terminator row where the DFS reaches it.  The bottom-up pass counts the
+// inner nodes and Woop rows under every node, which fixes every subtree's
+// place, so subtrees are written in parallel.
+struct Bvh2Layout {
+    const cwbvh_mesh* mesh;
+    const Tree& T;
+    cwbvh_layout& L;
+    std::vector<uint8_t> leaf;
+    std::vector<int32_t> sub_nodes, sub_rows;
+    bool dup = false;  // spatial splits: a leaf can name a triangle twice
+    Forker fork;
+    std::atomic<int64_t> leaves{0}, tris{0};
+
+    Bvh2Layout(const cwbvh_mesh* m, const Tree& t, cwbvh_layout& l, int threads)
+        : mesh(m), T(t), L(l), fork(threads) {}
+
+    void leaf_prims(int32_t n, std::vector<int32_t>& out) const {
+        thread_local std::vector<int32_t> scratch;
+        out.clear();
+        gather_prims(T, n, scratch, out);
+        if (dup) unique_in_order(out);
+    }
+
+    // rows of leaf n at `row`: its triangles, then the terminator
+    void write_leaf(int32_t n, int64_t row) {
+        thread_local std::vector<int32_t> prims;
+        leaf_prims(n, prims);
+        for (size_t k = 0; k < prims.size(); ++k) write_triangle(mesh, prims[k], row + 3 * (int64_t)k, L);
+        const int64_t end = row + 3 * (int64_t)prims.size();
+        L.woop[(size_t)end] = cwbvh_float4{u2f(0x80000000u), 0.f, 0.f, 0.f};
+        L.prim_idx[(size_t)end] = -1;
+        leaves.fetch_add(1, std::memory_order_relaxed);
+        tris.fetch_add((int64_t)prims.size(), std::memory_order_relaxed);
+    }
+
+    // inner node n at float4 `addr`, its subtree's rows from `row`; returns
+    // the depth of the deepest inner node below
+    int write_node(int32_t n, int64_t addr, int64_t row, int depth) {
+        const BNode& nd = T.nodes[n];
+        const int32_t kid[2] = {nd.left, nd.right};
+        int deepest[2] = {depth, depth};
+        std::vector<std::thread> group;
+        int64_t a = addr + 4, r = row;
+        for (int slot = 0; slot < 2; ++slot) {
+            const int32_t c = kid[slot];
+            bvh2_put_box(L, addr, slot, T.nodes[c].box);
+            if (leaf[c]) {
+                bvh2_put_child(L, addr, slot, ~(int32_t)r);
+                write_leaf(c, r);
+            } else {
+                bvh2_put_child(L, addr, slot, (int32_t)a);
+                int* out = &deepest[slot];
+                fork.run(T.nodes[c].prims >= Forker::kMinPrims, group,
+                         [this, c, a, r, depth, out] { *out = write_node(c, a, r, depth + 1); });
+            }
+            a += 4 * (int64_t)sub_nodes[c];
+            r += sub_rows[c];
+        }
+        Forker::join(group);
+        return std::max(deepest[0], deepest[1]);
+    }
+};
+
 void layout_bvh2(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params& P, cwbvh_layout& L) {
     const auto& N = T.nodes;
-    const int threads = num_threads_or_default(P.num_threads);
-    int32_t nn = (int32_t)N.size();
+    const int32_t nn = (int32_t)N.size();
+    Bvh2Layout B(mesh, T, L, num_threads_or_default(P.num_threads));
+    B.dup = T.refs.size() > (size_t)mesh->nt();
     // bottom-up SAH: make a leaf wherever it is cheaper (children indices > parent)
     std::vector<double> cost(nn);
-    std::vector<uint8_t> leaf(nn, 0);
+    B.leaf.assign(nn, 0);
+    B.sub_nodes.assign(nn, 0);
+    B.sub_rows.assign(nn, 0);
+    std::vector<int32_t> prims;
     for (int32_t i = nn - 1; i >= 0; --i) {
         const BNode& nd = N[i];
         double A = nd.box.area();
         double lc = A * nd.prims * P.bvh2_tri_cost;
-        if (nd.left < 0) {
+        bool is_leaf = nd.left < 0;
+        if (is_leaf) {
             cost[i] = lc;
-            leaf[i] = 1;
-            continue;
-        }
-        double ic = A * P.bvh2_node_cost + cost[nd.left] + cost[nd.right];
-        if (nd.prims <= P.bvh2_max_leaf && lc <= ic) {
-            cost[i] = lc;
-            leaf[i] = 1;
         } else {
-            cost[i] = ic;
+            double ic = A * P.bvh2_node_cost + cost[nd.left] + cost[nd.right];
+            is_leaf = nd.prims <= P.bvh2_max_leaf && lc <= ic;
+            cost[i] = is_leaf ? lc : ic;
+        }
+        B.leaf[i] = is_leaf ? 1 : 0;
+        if (is_leaf) {
+            int32_t cnt = nd.prims;
+            if (B.dup) {
+                B.leaf_prims(i, prims);
+                cnt = (int32_t)prims.size();
+            }
+            B.sub_rows[i] = 3 * cnt + 1;
+        } else {
+            B.sub_nodes[i] = 1 + B.sub_nodes[nd.left] + B.sub_nodes[nd.right];
+            B.sub_rows[i] = B.sub_rows[nd.left] + B.sub_rows[nd.right];
         }
     }
     double root_area = N[T.root].box.area();
@@ -750,86 +861,43 @@ void layout_bvh2(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params
 
     L.kind = CWBVH_KIND_BVH2;
     L.node_stride = 4;
-    const int64_t nt = (int64_t)T.refs.size();
-    std::vector<TriSlot> slots;
-    slots.reserve((size_t)nt);
-    L.woop.reserve((size_t)(4 * nt + 2));
-    L.prim_idx.reserve((size_t)(4 * nt + 2));
-    std::vector<int32_t> scratch, prims;
-    auto gather = [&](int32_t n, std::vector<int32_t>& out) {
-        out.clear();
-        gather_prims(T, n, scratch, out);
-        unique_in_order(out);
-    };
-    auto emit_leaf = [&](const std::vector<int32_t>& prims) -> int32_t {
-        int32_t addr = (int32_t)L.woop.size();
-        for (int32_t p : prims) reserve_tri(p, L, slots);
-        L.woop.push_back(cwbvh_float4{u2f(0x80000000u), 0.f, 0.f, 0.f});
-        L.prim_idx.push_back(-1);
-        L.num_leaves++;
-        return ~addr;
-    };
-    auto put_child = [&](int32_t node_addr, int slot, int32_t value) {
-        if (slot == 0) L.nodes[node_addr + 3].x = i2f(value);
-        else L.nodes[node_addr + 3].y = i2f(value);
-    };
-    auto put_box = [&](int32_t node_addr, int slot, const Box& b) {
-        if (slot == 0) {
-            L.nodes[node_addr + 0] = cwbvh_float4{b.lo.x, b.hi.x, b.lo.y, b.hi.y};
-            L.nodes[node_addr + 2].x = b.lo.z;
-            L.nodes[node_addr + 2].y = b.hi.z;
-        } else {
-            L.nodes[node_addr + 1] = cwbvh_float4{b.lo.x, b.hi.x, b.lo.y, b.hi.y};
-            L.nodes[node_addr + 2].z = b.lo.z;
-            L.nodes[node_addr + 2].w = b.hi.z;
-        }
-    };
-    auto new_node = [&]() {
-        int32_t a = (int32_t)L.nodes.size();
-        for (int k = 0; k < 4; ++k) L.nodes.push_back(cwbvh_float4{0.f, 0.f, 0.f, 0.f});
-        L.num_nodes++;
-        return a;
-    };
-    Box inverted;
-    inverted.lo = V3{FLT_MAX, FLT_MAX, FLT_MAX};
-    inverted.hi = V3{-FLT_MAX, -FLT_MAX, -FLT_MAX};
-
     const BNode& root = N[T.root];
-    if (root.prims == 0 || leaf[T.root]) {
-        // wrap: inner root whose second child can never be hit
-        int32_t r = new_node();
-        if (root.prims) gather(T.root, prims);
-        put_box(r, 0, root.prims ? root.box : inverted);
-        put_child(r, 0, emit_leaf(prims));
-        put_box(r, 1, inverted);
-        put_child(r, 1, emit_leaf({}));
-        fill_woop(mesh, slots, threads, L);
+    if (root.prims == 0 || B.leaf[T.root]) {
+        // wrap: inner root whose first child is the leaf (or an empty leaf) and
+        // whose second child is an empty leaf that can never be hit
+        Box inverted;
+        inverted.lo = V3{FLT_MAX, FLT_MAX, FLT_MAX};
+        inverted.hi = V3{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+        const int64_t rows = root.prims ? B.sub_rows[T.root] : 1;
+        L.nodes.assign(4, cwbvh_float4{0.f, 0.f, 0.f, 0.f});
+        L.woop.assign((size_t)rows + 1, cwbvh_float4{0.f, 0.f, 0.f, 0.f});
+        L.prim_idx.assign((size_t)rows + 1, -1);
+        bvh2_put_box(L, 0, 0, root.prims ? root.box : inverted);
+        bvh2_put_child(L, 0, 0, ~0);
+        if (root.prims) {
+            B.write_leaf(T.root, 0);
+        } else {
+            L.woop[0] = cwbvh_float4{u2f(0x80000000u), 0.f, 0.f, 0.f};
+            B.leaves.fetch_add(1);
+        }
+        bvh2_put_box(L, 0, 1, inverted);
+        bvh2_put_child(L, 0, 1, ~(int32_t)rows);
+        L.woop[(size_t)rows] = cwbvh_float4{u2f(0x80000000u), 0.f, 0.f, 0.f};
+        B.leaves.fetch_add(1);
+        L.num_nodes = 1;
+        L.num_leaves = B.leaves.load();
+        L.num_tri_refs = B.tris.load();
         L.max_depth = 1;
         L.stack_need = 2;
         return;
     }
-    // pre-order DFS; stack holds (tree node, parent address, slot, depth)
-    struct Item { int32_t n, parent, slot, depth; };
-    std::vector<Item> st{{T.root, -1, 0, 1}};
-    int32_t maxd = 0;
-    while (!st.empty()) {
-        Item it = st.back();
-        st.pop_back();
-        const BNode& nd = N[it.n];
-        if (leaf[it.n]) {
-            gather(it.n, prims);
-            put_child(it.parent, it.slot, emit_leaf(prims));
-            continue;
-        }
-        int32_t a = new_node();
-        maxd = std::max(maxd, it.depth);
-        if (it.parent >= 0) put_child(it.parent, it.slot, a);
-        put_box(a, 0, N[nd.left].box);
-        put_box(a, 1, N[nd.right].box);
-        st.push_back({nd.right, a, 1, it.depth + 1});
-        st.push_back({nd.left, a, 0, it.depth + 1});
-    }
-    fill_woop(mesh, slots, threads, L);
+    L.nodes.assign(4 * (size_t)B.sub_nodes[T.root], cwbvh_float4{0.f, 0.f, 0.f, 0.f});
+    L.woop.assign((size_t)B.sub_rows[T.root], cwbvh_float4{0.f, 0.f, 0.f, 0.f});
+    L.prim_idx.assign((size_t)B.sub_rows[T.root], -1);
+    const int maxd = B.write_node(T.root, 0, 0, 1);
+    L.num_nodes = B.sub_nodes[T.root];
+    L.num_leaves = B.leaves.load();
+    L.num_tri_refs = B.tris.load();
     L.max_depth = maxd;
     L.stack_need = maxd + 1;
 }
@@ -845,7 +913,12 @@ struct Collapse {
 
     Collapse(const Tree& t, const cwbvh_build_params& p) : T(t), P(p) {}
 
-    void run() {
+    double inv_root = 0.0;
+
+    // The DP of every node, children before parents: a post-order walk
+    // whose large subtrees run on other threads (each node's entries depend
+    // on its children's only, so the table is the same for any thread count).
+    void run(int threads) {
         const auto& N = T.nodes;
         int32_t nn = (int32_t)N.size();
         C.assign((size_t)nn * 8, 0.f);
@@ -854,80 +927,99 @@ struct Collapse {
         // areas relative to the root keep the float cost table finite for any
         // coordinate range
         const double root_area = nn ? N[T.root].box.area() : 0.0;
-        const double inv_root = root_area > 0.0 && std::isfinite(root_area) ? 1.0 / root_area : 0.0;
-        for (int32_t n = nn - 1; n >= 0; --n) {
-            const BNode& nd = N[n];
-            const float A = inv_root > 0.0 ? (float)(nd.box.area() * inv_root) : 0.f;
-            float* c = &C[(size_t)n * 8];
-            uint8_t* d = &dec[(size_t)n * 8];
-            if (nd.left < 0) {
-                float lc = A * nd.prims * P.cwbvh_tri_cost;
-                for (int i = 1; i < 8; ++i) c[i] = lc;
-                d[1] = 1;
-                continue;
-            }
-            const float* cl = &C[(size_t)nd.left * 8];
-            const float* cr = &C[(size_t)nd.right * 8];
-            float dist[9];
-            uint8_t dk[9];
-            for (int j = 2; j <= 8; ++j) {
-                float best = INFINITY;
-                uint8_t bk = 1;
-                for (int k = 1; k < j; ++k) {
-                    float v = cl[k] + cr[j - k];
-                    if (v < best) {
-                        best = v;
-                        bk = (uint8_t)k;
-                    }
+        inv_root = root_area > 0.0 && std::isfinite(root_area) ? 1.0 / root_area : 0.0;
+        if (nn == 0) return;
+        Forker fork(threads);
+        solve(T.root, fork);
+    }
+
+    void solve(int32_t n, Forker& fork) {
+        const BNode& nd = T.nodes[n];
+        if (nd.left >= 0) {
+            std::vector<std::thread> group;
+            const int32_t l = nd.left;
+            fork.run(T.nodes[l].prims >= Forker::kMinPrims, group, [this, l, &fork] { solve(l, fork); });
+            solve(nd.right, fork);
+            Forker::join(group);
+        }
+        solve_node(n);
+    }
+
+    void solve_node(int32_t n) {
+        const auto& N = T.nodes;
+        const BNode& nd = N[n];
+        const float A = inv_root > 0.0 ? (float)(nd.box.area() * inv_root) : 0.f;
+        float* c = &C[(size_t)n * 8];
+        uint8_t* d = &dec[(size_t)n * 8];
+        if (nd.left < 0) {
+            float lc = A * nd.prims * P.cwbvh_tri_cost;
+            for (int i = 1; i < 8; ++i) c[i] = lc;
+            d[1] = 1;
+            return;
+        }
+        const float* cl = &C[(size_t)nd.left * 8];
+        const float* cr = &C[(size_t)nd.right * 8];
+        float dist[9];
+        uint8_t dk[9];
+        for (int j = 2; j <= 8; ++j) {
+            float best = INFINITY;
+            uint8_t bk = 1;
+            for (int k = 1; k < j; ++k) {
+                float v = cl[k] + cr[j - k];
+                if (v < best) {
+                    best = v;
+                    bk = (uint8_t)k;
                 }
-                dist[j] = best;
-                dk[j] = bk;
             }
-            float inner = A * P.cwbvh_node_cost + dist[8];
-            float leafc = nd.prims <= 3 ? A * nd.prims * P.cwbvh_tri_cost : INFINITY;
-            // a leaf slot holds at most 3 triangles (unary count in the meta
-            // byte), whatever the costs say
-            if (nd.prims <= 3 && leafc <= inner) {
-                c[1] = leafc;
-                d[1] = 1;
+            dist[j] = best;
+            dk[j] = bk;
+        }
+        float inner = A * P.cwbvh_node_cost + dist[8];
+        float leafc = nd.prims <= 3 ? A * nd.prims * P.cwbvh_tri_cost : INFINITY;
+        // a leaf slot holds at most 3 triangles (unary count in the meta
+        // byte), whatever the costs say
+        if (nd.prims <= 3 && leafc <= inner) {
+            c[1] = leafc;
+            d[1] = 1;
+        } else {
+            c[1] = inner;
+            d[1] = 0;
+        }
+        split8[n] = dk[8];
+        for (int i = 2; i < 8; ++i) {
+            if (dist[i] < c[i - 1]) {
+                c[i] = dist[i];
+                d[i] = dk[i];
             } else {
-                c[1] = inner;
-                d[1] = 0;
-            }
-            split8[n] = dk[8];
-            for (int i = 2; i < 8; ++i) {
-                if (dist[i] < c[i - 1]) {
-                    c[i] = dist[i];
-                    d[i] = dk[i];
-                } else {
-                    c[i] = c[i - 1];
-                    d[i] = 0;
-                }
+                c[i] = c[i - 1];
+                d[i] = 0;
             }
         }
     }
 
-    // children of the 8-wide node made from binary node n
-    void collect(int32_t n, int i, std::vector<int32_t>& out) const {
+    // children of the 8-wide node made from binary node n, left to right
+    void collect(int32_t n, int i, int32_t out[8], int& cnt) const {
         const BNode& nd = T.nodes[n];
         while (i > 1 && dec[(size_t)n * 8 + i] == 0) --i;
         if (i == 1 || nd.left < 0) {
-            out.push_back(n);
+            out[cnt++] = n;
             return;
         }
         int k = dec[(size_t)n * 8 + i];
-        collect(nd.left, k, out);
-        collect(nd.right, i - k, out);
+        collect(nd.left, k, out, cnt);
+        collect(nd.right, i - k, out, cnt);
     }
-    void children_of(int32_t n, std::vector<int32_t>& out) const {
+    int children_of(int32_t n, int32_t out[8]) const {
         const BNode& nd = T.nodes[n];
+        int cnt = 0;
         if (nd.left < 0) {
-            out.push_back(n);
-            return;
+            out[cnt++] = n;
+            return cnt;
         }
         int k = split8[n];
-        collect(nd.left, k, out);
-        collect(nd.right, 8 - k, out);
+        collect(nd.left, k, out, cnt);
+        collect(nd.right, 8 - k, out, cnt);
+        return cnt;
     }
     bool is_leaf(int32_t n) const { return T.nodes[n].left < 0 || dec[(size_t)n * 8 + 1] == 1; }
 };
@@ -942,8 +1034,7 @@ int exponent_for(double extent) {
     return e;
 }
 
-void assign_slots(const Box& nb, const std::vector<Box>& cbox, int slot_of[8]) {
-    int nc = (int)cbox.size();
+void assign_slots(const Box& nb, const Box* cbox, int nc, int slot_of[8]) {
     V3 nc_c = nb.center();
     struct Cand { float cost; int s, c; };
     // at most 8 slots x 8 children, sorted by cost with ties kept in
@@ -987,56 +1078,86 @@ inline int quant_ceil(double x) {
     return t + ((double)t < x ? 1 : 0);
 }
 
-void layout_cwbvh(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params& P, cwbvh_layout& L) {
-    Collapse CL(T, P);
-    if (!T.nodes.empty() && T.nodes[T.root].prims > 0) CL.run();
-    const int S = P.cwbvh_node_stride;
-    L.kind = CWBVH_KIND_CWBVH;
-    L.node_stride = S;
-    auto new_nodes = [&](int count) {
-        int32_t idx = (int32_t)(L.nodes.size() / S);
-        L.nodes.resize(L.nodes.size() + (size_t)count * S, cwbvh_float4{0.f, 0.f, 0.f, 0.f});
-        L.num_nodes += count;
-        return idx;
-    };
-    const int threads = num_threads_or_default(P.num_threads);
-    const int64_t nt = (int64_t)T.refs.size();
-    std::vector<TriSlot> slots;
-    slots.reserve((size_t)nt);
-    L.woop.reserve((size_t)(3 * nt));
-    L.prim_idx.reserve((size_t)(3 * nt));
-    std::vector<int32_t> scratch, prims;
-    double sah = 0.0;
-    double root_area = T.nodes.empty() ? 0.0 : T.nodes[T.root].box.area();
+// The CWBVH layout in the order of a DFS that takes a node's inner children
+// in reverse slot order (the order a stack pushed in slot order pops them):
+// a node's inner children occupy consecutive node slots, allocated when the
+// node is reached, and its leaf triangles consecutive Woop rows, appended
+// when it is reached.  A first pass counts, under every 8-wide node, the
+// nodes its subtree allocates and the triangles it appends; those counts fix
+// where every subtree lands, so the second pass writes subtrees in parallel.
+struct CwbvhLayout {
+    const cwbvh_mesh* mesh;
+    const Tree& T;
+    const Collapse& CL;
+    const cwbvh_build_params& P;
+    cwbvh_layout& L;
+    const int S;
+    double root_area = 0.0;
+    bool dup = false;  // spatial splits: a leaf can name a triangle twice
+    // per binary node that roots an 8-wide node: nodes and triangles its
+    // subtree allocates (its inner children and everything below them)
+    std::vector<int32_t> sub_nodes, sub_tris;
+    std::vector<double> sah_part;  // per output node, summed in node order
+    std::atomic<int64_t> leaves{0};
+    Forker fork;
 
-    new_nodes(1);
-    if (T.nodes.empty() || T.nodes[T.root].prims == 0) {
-        L.max_depth = 1;
-        L.stack_need = 2;
-        L.sah_cost = 0.0;
-        return;
+    CwbvhLayout(const cwbvh_mesh* m, const Tree& t, const Collapse& cl, const cwbvh_build_params& p,
+                cwbvh_layout& l, int threads)
+        : mesh(m), T(t), CL(cl), P(p), L(l), S(p.cwbvh_node_stride), fork(threads) {}
+
+    void leaf_prims(int32_t n, std::vector<int32_t>& out) const {
+        thread_local std::vector<int32_t> scratch;
+        out.clear();
+        gather_prims(T, n, scratch, out);
+        if (dup) unique_in_order(out);
     }
-    struct Item { int32_t n, node_index, depth; };
-    std::vector<Item> st{{T.root, 0, 1}};
-    int32_t maxd = 0;
-    std::vector<int32_t> kids;
-    std::vector<Box> cbox;
-    while (!st.empty()) {
-        Item it = st.back();
-        st.pop_back();
-        maxd = std::max(maxd, it.depth);
-        kids.clear();
-        CL.children_of(it.n, kids);
-        int nc = (int)kids.size();
-        cbox.clear();
-        Box nb;
-        for (int32_t k : kids) {
-            cbox.push_back(T.nodes[k].box);
-            nb.grow(T.nodes[k].box);
+    int32_t leaf_count(int32_t n) const {
+        if (!dup) return T.nodes[n].prims;
+        thread_local std::vector<int32_t> prims;
+        leaf_prims(n, prims);
+        return (int32_t)prims.size();
+    }
+
+    void count(int32_t b) {
+        int32_t kids[8];
+        const int nc = CL.children_of(b, kids);
+        int32_t n_inner = 0, tris = 0;
+        std::vector<std::thread> group;
+        for (int c = 0; c < nc; ++c) {
+            const int32_t k = kids[c];
+            if (CL.is_leaf(k)) {
+                tris += leaf_count(k);
+            } else {
+                ++n_inner;
+                fork.run(T.nodes[k].prims >= Forker::kMinPrims, group, [this, k] { count(k); });
+            }
         }
-        if (root_area > 0) sah += P.cwbvh_node_cost * nb.area() / root_area;
+        Forker::join(group);
+        int32_t nodes = n_inner;
+        for (int c = 0; c < nc; ++c) {
+            if (CL.is_leaf(kids[c])) continue;
+            nodes += sub_nodes[kids[c]];
+            tris += sub_tris[kids[c]];
+        }
+        sub_nodes[b] = nodes;
+        sub_tris[b] = tris;
+    }
+
+    // 8-wide node made from binary node b at node slot `index`; its inner
+    // children take node slots from `alloc`, its triangles Woop triangle
+    // slots from `wo`.  Returns the depth of the deepest node below.
+    int write(int32_t b, int32_t index, int32_t alloc, int32_t wo, int depth) {
+        int32_t kids[8];
+        const int nc = CL.children_of(b, kids);
+        Box cbox[8];
+        Box nb;
+        for (int c = 0; c < nc; ++c) {
+            cbox[c] = T.nodes[kids[c]].box;
+            nb.grow(cbox[c]);
+        }
+        double sah = root_area > 0 ? P.cwbvh_node_cost * nb.area() / root_area : 0.0;
         int slot_of[8];
-        assign_slots(nb, cbox, slot_of);
+        assign_slots(nb, cbox, nc, slot_of);
         int32_t child_at[8];
         for (int s = 0; s < 8; ++s) child_at[s] = -1;
         for (int c = 0; c < nc; ++c) child_at[slot_of[c]] = c;
@@ -1044,16 +1165,9 @@ void layout_cwbvh(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_param
         int ex[3];
         for (int a = 0; a < 3; ++a) ex[a] = exponent_for((double)comp(nb.hi, a) - (double)comp(nb.lo, a));
         uint8_t imask = 0;
-        int inner_count = 0, tri_count = 0;
-        int32_t child_base = 0, tri_base = 0;
-        // inner children are allocated contiguously, in slot order, before
-        // any pointer into the node array is taken
-        int n_inner = 0;
-        for (int s = 0; s < 8; ++s)
-            if (child_at[s] >= 0 && !CL.is_leaf(kids[child_at[s]])) ++n_inner;
-        if (n_inner) child_base = new_nodes(n_inner);
-        tri_base = (int32_t)(L.woop.size() / 3);
-        cwbvh_float4* node = &L.nodes[(size_t)it.node_index * S];
+        int32_t inner_count = 0, tri_count = 0;
+        int32_t child_index[8];
+        cwbvh_float4* node = &L.nodes[(size_t)index * S];
         uint8_t* n0b = reinterpret_cast<uint8_t*>(&node[0]);
         uint8_t* meta = reinterpret_cast<uint8_t*>(&node[1]) + 8;
         uint8_t* q = reinterpret_cast<uint8_t*>(&node[2]);
@@ -1062,41 +1176,97 @@ void layout_cwbvh(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_param
         node[0].z = nb.lo.z;
         double scale[3];
         for (int a = 0; a < 3; ++a) scale[a] = std::ldexp(1.0, -ex[a]);
+        thread_local std::vector<int32_t> prims;
         for (int s = 0; s < 8; ++s) {
-            int c = child_at[s];
+            child_index[s] = -1;
+            const int c = child_at[s];
             if (c < 0) continue;
-            const Box& b = cbox[c];
+            const Box& bx = cbox[c];
             for (int a = 0; a < 3; ++a) {
-                q[a * 8 + s] = (uint8_t)quant_floor(((double)comp(b.lo, a) - (double)comp(nb.lo, a)) * scale[a]);
-                q[24 + a * 8 + s] = (uint8_t)quant_ceil(((double)comp(b.hi, a) - (double)comp(nb.lo, a)) * scale[a]);
+                q[a * 8 + s] = (uint8_t)quant_floor(((double)comp(bx.lo, a) - (double)comp(nb.lo, a)) * scale[a]);
+                q[24 + a * 8 + s] = (uint8_t)quant_ceil(((double)comp(bx.hi, a) - (double)comp(nb.lo, a)) * scale[a]);
             }
-            int32_t bn = kids[c];
+            const int32_t bn = kids[c];
             if (!CL.is_leaf(bn)) {
                 imask |= (uint8_t)(1u << s);
                 meta[s] = (uint8_t)((1u << 5) | (24u + (unsigned)s));
-                st.push_back({bn, child_base + inner_count, it.depth + 1});
-                ++inner_count;
+                child_index[s] = alloc + inner_count++;
             } else {
-                prims.clear();
-                gather_prims(T, bn, scratch, prims);
-                unique_in_order(prims);
-                int cnt = (int)prims.size();
-                unsigned unary = cnt == 1 ? 1u : (cnt == 2 ? 3u : 7u);
+                leaf_prims(bn, prims);
+                const int cnt = (int)prims.size();
+                const unsigned unary = cnt == 1 ? 1u : (cnt == 2 ? 3u : 7u);
                 meta[s] = (uint8_t)((unary << 5) | (unsigned)tri_count);
+                for (int k = 0; k < cnt; ++k)
+                    write_triangle(mesh, prims[(size_t)k], 3 * ((int64_t)wo + tri_count + k), L);
                 tri_count += cnt;
-                for (int32_t p : prims) reserve_tri(p, L, slots);
-                L.num_leaves++;
-                if (root_area > 0) sah += P.cwbvh_tri_cost * cnt * b.area() / root_area;
+                leaves.fetch_add(1, std::memory_order_relaxed);
+                if (root_area > 0) sah += P.cwbvh_tri_cost * cnt * bx.area() / root_area;
             }
         }
         n0b[12] = (uint8_t)(int8_t)ex[0];
         n0b[13] = (uint8_t)(int8_t)ex[1];
         n0b[14] = (uint8_t)(int8_t)ex[2];
         n0b[15] = imask;
-        node[1].x = i2f(child_base);
-        node[1].y = i2f(tri_base);
+        node[1].x = i2f(inner_count ? alloc : 0);
+        node[1].y = i2f(wo);
+        sah_part[(size_t)index] = sah;
+
+        // inner children, highest slot first, each with the allocations of
+        // the subtrees taken before it
+        int deepest[8];
+        std::vector<std::thread> group;
+        int32_t a_next = alloc + inner_count, w_next = wo + tri_count;
+        for (int s = 7; s >= 0; --s) {
+            deepest[s] = depth;
+            if (child_index[s] < 0) continue;
+            const int32_t bn = kids[child_at[s]];
+            const int32_t ci = child_index[s], ca = a_next, cw = w_next;
+            int* out = &deepest[s];
+            fork.run(T.nodes[bn].prims >= Forker::kMinPrims, group,
+                     [this, bn, ci, ca, cw, depth, out] { *out = write(bn, ci, ca, cw, depth + 1); });
+            a_next += sub_nodes[bn];
+            w_next += sub_tris[bn];
+        }
+        Forker::join(group);
+        int d = depth;
+        for (int s = 0; s < 8; ++s) d = std::max(d, deepest[s]);
+        return d;
     }
-    fill_woop(mesh, slots, threads, L);
+};
+
+void layout_cwbvh(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params& P, cwbvh_layout& L) {
+    const int threads = num_threads_or_default(P.num_threads);
+    const int S = P.cwbvh_node_stride;
+    L.kind = CWBVH_KIND_CWBVH;
+    L.node_stride = S;
+    if (T.nodes.empty() || T.nodes[T.root].prims == 0) {
+        L.nodes.assign((size_t)S, cwbvh_float4{0.f, 0.f, 0.f, 0.f});
+        L.num_nodes = 1;
+        L.max_depth = 1;
+        L.stack_need = 2;
+        L.sah_cost = 0.0;
+        return;
+    }
+    Collapse CL(T, P);
+    CL.run(threads);
+    CwbvhLayout W(mesh, T, CL, P, L, threads);
+    W.root_area = T.nodes[T.root].box.area();
+    W.dup = T.refs.size() > (size_t)mesh->nt();
+    W.sub_nodes.assign(T.nodes.size(), 0);
+    W.sub_tris.assign(T.nodes.size(), 0);
+    W.count(T.root);
+    const int64_t num_nodes = 1 + (int64_t)W.sub_nodes[T.root];
+    const int64_t num_tris = W.sub_tris[T.root];
+    L.nodes.assign((size_t)(num_nodes * S), cwbvh_float4{0.f, 0.f, 0.f, 0.f});
+    L.woop.assign((size_t)(3 * num_tris), cwbvh_float4{0.f, 0.f, 0.f, 0.f});
+    L.prim_idx.assign((size_t)(3 * num_tris), -1);
+    W.sah_part.assign((size_t)num_nodes, 0.0);
+    const int maxd = W.write(T.root, 0, 1, 0, 1);
+    double sah = 0.0;
+    for (double v : W.sah_part) sah += v;
+    L.num_nodes = num_nodes;
+    L.num_leaves = W.leaves.load();
+    L.num_tri_refs = num_tris;
     L.max_depth = maxd;
     L.stack_need = maxd + 1;
     L.sah_cost = sah;
