@@ -35,6 +35,8 @@ def main() -> int:
         wl = W.c0()
     elif args.workload == "c3":
         wl = W.c3(args.c3_tris, args.c3_rays)
+        # the benchmark's shuffled order
+        wl.rays = cw.shuffle_rays(wl.rays, seed=W.SEEDS["C3"] + 2)
     else:
         wl = W.c1(W.gpu_primary_tracer())
     cfg = None
@@ -59,6 +61,7 @@ def main() -> int:
     cw.synchronize()
     print(json.dumps({"workload": wl.name, "rays": n, "copy_bytes": copy_bytes, "launches": args.launches,
                       "flags": flags, "cwbvh_nodes": cwb.stats()["num_nodes"], "loop_mode": args.loop_mode,
+                      "node_bytes": 16 * s_cw.node_stride, "triangles": wl.mesh.num_triangles,
                       "scene_bytes": {"cwbvh": s_cw.device_bytes(), "bvh2": s_b2.device_bytes()}}))
     return 0
 
