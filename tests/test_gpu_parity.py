@@ -369,6 +369,52 @@ def test_edge_cases(cw, orc):
         cw.synchronize()
 
 
+def _pathological_rays():
+    """Rays from above a heightfield, 64 per kind: NaN origin, NaN direction,
+    zero direction, tmin > tmax, negative tmin, tmin = -0, huge origins,
+    NaN tmax, NaN tmin, a direction of length 1e-30."""
+    rng = np.random.default_rng(12)
+    r = np.zeros((640, 8), np.float32)
+    r[:, :3] = rng.uniform(-0.2, 1.2, size=(640, 3))
+    r[:, 2] = 1.0
+    d = rng.normal(size=(640, 3))
+    d[:, 2] = -np.abs(d[:, 2]) - 0.5
+    r[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r[:, 7] = np.inf
+    nan = np.float32(np.nan)
+    r[0:64, 0] = nan
+    r[64:128, 5] = nan
+    r[128:192, 4:7] = 0.0
+    r[192:256, 3], r[192:256, 7] = 5.0, 1.0
+    r[256:320, 3] = -1.0
+    r[320:384, 3] = -0.0
+    r[384:448, :3] *= 1e30
+    r[448:512, 7] = nan
+    r[512:576, 3] = nan
+    r[576:640, 4:7] = [0.0, 0.0, -1e-30]
+    return r
+
+
+@pytest.mark.parametrize("kw", [None, dict(loop_mode=0), dict(loop_mode=2), dict(persistent=0)])
+def test_pathological_rays(cw, orc, kw):
+    """NaN, zero-length and inverted-interval rays terminate and return what
+    the oracle returns, bit for bit (NaN rays miss)."""
+    mesh = cw.Mesh.heightfield(32, 16, 8, 0.1, seed=2)
+    bvh2, cwb = cw.build_layouts(mesh)
+    rays = _pathological_rays()
+    for lay in (cwb, bvh2):
+        if kw is not None and lay.kind != cw.KIND_CWBVH:
+            continue
+        sc = cw.Scene(lay)
+        cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw) if kw is not None else None
+        for any_hit in (False, True):
+            say(f"pathological rays kind={lay.kind} {kw} any={any_hit}")
+            g = sc.trace_numpy(rays, int(any_hit), cfg)
+            ref = oracle_trace(orc, lay, rays, any_hit)
+            assert_bitwise(g, ref, f"pathological kind={lay.kind} any={any_hit}")
+            assert np.all(orc.hit_ids(ref)[:128] == -1)
+
+
 def test_single_triangle_and_empty_scene(cw, orc):
     one = cw.Mesh.from_arrays([[0, 0, 0], [1, 0, 0], [0, 1, 0]], [[0, 1, 2]])
     empty = cw.Mesh.from_arrays(np.zeros((0, 3), np.float32), np.zeros((0, 3), np.int32))
