@@ -149,6 +149,20 @@ def bench_c2(args, cw, W, dev) -> int:
     untuned = cw.Batch(items, cw.TRACE_CLOSEST, cfg0, morton=False)
     ms_untuned = timed(lambda: untuned.launch(st), args.steps, args.warmup)
     value = total / (ms_batch / 1e3) / 1e6
+    # algorithmic traffic of the batch (counting launches per scene, outside
+    # the timed region): every scene's rays, hits, node visits and triangle
+    # tests in reference-layout bytes
+    d_cnt = cw.DeviceBuffer(16)
+    nodes_c = tris_c = 0
+    for sc, r, h, n in items:
+        d_cnt.zero()
+        sc.trace(r, h, n, cw.TRACE_STATS, cfg0, d_cnt.ptr, st)
+        st.synchronize()
+        c = d_cnt.download((2,), np.uint64)
+        nodes_c += int(c[0])
+        tris_c += int(c[1])
+    bytes_c = algorithmic_bytes(1, total, nodes_c, tris_c)
+    achieved = bytes_c / (ms_batch / 1e3) / 1e9
     out = {
         "metric": METRIC + " [C2 batched]",
         "value": round(value, 2),
@@ -169,6 +183,12 @@ def bench_c2(args, cw, W, dev) -> int:
         "plain_mrays_s": round(total / ms_plain / 1e3, 2),
         "plain_sync_ms": round(ms_plain_sync, 4),
         "batched_untuned_ms": round(ms_untuned, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "cwbvh_trace<closest, MULTI> (autotuned persistent batch, graph replay)",
+                     "algorithmic_bytes_per_launch": bytes_c, "avg_launch_ms": round(ms_batch, 4),
+                     "per_ray": {"nodes": nodes_c / total, "triangles": tris_c / total, "bytes": bytes_c / total},
+                     "plain_achieved": round(bytes_c / (ms_plain / 1e3) / 1e9, 1)},
         "speedup_vs_plain": round(ms_plain / ms_batch, 3),
         "speedup_vs_plain_sync": round(ms_plain_sync / ms_batch, 3),
         "autotune": {"best": best.as_dict(), "seconds": round(tune_s, 1),
@@ -350,6 +370,17 @@ def bench_ao(args, cw, W, dev) -> int:
     cw.synchronize()
     identical = bool(np.array_equal(fused.view(np.uint32), d_hits.download((n, 4), np.float32).view(np.uint32)))
     best = max(table.items(), key=lambda kv: kv[1]["mrays_per_s"])
+    # algorithmic traffic of the reported kernel (fused, per-surfel counts,
+    # any hit): 36 B of surfel read and 4 B of count per surfel, plus the node
+    # visits and triangle tests of the same rays traced as a buffer
+    d_cnt = cw.DeviceBuffer(16)
+    d_cnt.zero()
+    sc.trace(d_rays.ptr, d_hits.ptr, n, cw.TRACE_ANY | cw.TRACE_STATS, cfg, d_cnt.ptr)
+    cw.synchronize()
+    c = d_cnt.download((2,), np.uint64)
+    bytes_c = 40 * ntex + 80 * int(c[0]) + 48 * int(c[1])
+    ms_v = table["fused_counts_any"]["ms"]
+    achieved = bytes_c / (ms_v / 1e3) / 1e9
     out = {
         "metric": METRIC + " [AO from surfels, fused ray generation]",
         "value": table["fused_counts_any"]["mrays_per_s"],
@@ -370,6 +401,11 @@ def bench_ao(args, cw, W, dev) -> int:
         "speedup_fused_counts_vs_unfused_any": round(table["unfused_any"]["ms"] / table["fused_counts_any"]["ms"], 3),
         "speedup_fused_hits_vs_unfused_closest": round(table["unfused_closest"]["ms"] / table["fused_hits_closest"]["ms"], 3),
         "bytes_per_ray": {"unfused": 32 + 32 + 16, "fused_hits": 16, "fused_counts": "4 per surfel"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "cwbvh_trace<any, SRC=surfels> with per-surfel counts",
+                     "algorithmic_bytes_per_launch": bytes_c, "avg_launch_ms": ms_v,
+                     "per_ray": {"nodes": int(c[0]) / n, "triangles": int(c[1]) / n, "bytes": bytes_c / n}},
         "best": best[0],
         "device": dev["arch"],
     }
