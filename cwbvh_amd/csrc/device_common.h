@@ -73,10 +73,14 @@ __device__ __forceinline__ bool woop_test(const float4 v0, const float4 v1, cons
 // touches consecutive words); deeper entries spill to a per-lane scratch array.
 // The LDS array and the spill array are passed separately so that the index
 // state stays in registers and LDS is addressed with ds_* instructions.
+// LDS part of a traversal stack: each wave owns depth x 64 consecutive
+// entries, entry k of lane l at (k << 6) + l inside them, so a lane's entries
+// are one shift and one add away (no multiply) and the 64 lanes of a wave
+// touch 64 consecutive words per push or pop.
 struct StackIndex {
-    int col;     // threadIdx.x
-    int stride;  // blockDim.x
-    int depth;   // entries held in LDS
+    int col;    // (wave in block) x depth x 64 + lane
+    int depth;  // entries held in LDS
+    __device__ StackIndex(int thread, int depth_) : col(((thread >> 6) * depth_ << 6) + (thread & 63)), depth(depth_) {}
 };
 
 #define CWD_LDS __attribute__((address_space(3)))
@@ -88,7 +92,7 @@ __device__ __forceinline__ CWD_LDS T* lds_ptr(T* p) {
 
 template <class T, int CAP>
 __device__ __forceinline__ void stack_put(CWD_LDS T* lds, T (&spill)[CAP], const StackIndex& s, int sp, T v) {
-    if (sp < s.depth) lds[sp * s.stride + s.col] = v;
+    if (sp < s.depth) lds[(sp << 6) + s.col] = v;
     else spill[sp - s.depth] = v;
 }
 
@@ -96,7 +100,7 @@ __device__ __forceinline__ void stack_put(CWD_LDS T* lds, T (&spill)[CAP], const
 // cannot be merged into one generic (flat) load through a selected pointer.
 template <class T, int CAP>
 __device__ __forceinline__ T stack_get(const CWD_LDS T* lds, const T (&spill)[CAP], const StackIndex& s, int sp) {
-    if (sp < s.depth) return lds[sp * s.stride + s.col];
+    if (sp < s.depth) return lds[(sp << 6) + s.col];
     return spill[sp - s.depth];
 }
 

@@ -151,7 +151,7 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
     extern __shared__ unsigned long long lds_stack_raw[];
     CWD_LDS unsigned long long* lds_stack = lds_ptr(lds_stack_raw);
     unsigned long long spill[kStackCapacity];
-    const StackIndex sx{(int)threadIdx.x, (int)blockDim.x, P.lds_depth};
+    const StackIndex sx((int)threadIdx.x, P.lds_depth);
     const int lane = lane_id();
 
     // per-lane ray state
@@ -602,7 +602,7 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace(Bvh2Params P) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= P.n) return;
     int spill[kBvh2StackCapacity];
-    const StackIndex sx{(int)threadIdx.x, (int)blockDim.x, P.lds_depth};
+    const StackIndex sx((int)threadIdx.x, P.lds_depth);
 
     const int ri = P.order ? P.order[i] : i;
     const float4 ro = nt_load4(&P.rays[ri].o);
