@@ -92,6 +92,7 @@ class TraceConfig(C.Structure):
         ("work_slices", C.c_int32),
         ("persistent", C.c_int32),
         ("loop_mode", C.c_int32),
+        ("occupancy", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

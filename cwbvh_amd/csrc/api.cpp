@@ -413,6 +413,7 @@ int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* c) {
         c->work_slices = 8;
         c->persistent = 2;    // automatic by ray count
         c->loop_mode = 1;
+        c->occupancy = 0;     // automatic by ray count
     } else {
         c->block_size = 256;
         c->lds_depth = 8;

@@ -141,13 +141,27 @@ constexpr int kLoopGroup = 0;
 constexpr int kLoopInterleave = 1;
 constexpr int kLoopSpeculate = 2;
 
-// Occupancy hint for the register allocator (empty by default).
+// Register budget (cwbvh_trace_config.occupancy).  The plain interleaved
+// kernel (one scene, rays from a buffer, no counters, no prefetch) takes 75
+// VGPRs unconstrained: 6 waves per SIMD.  It is also built for 7 waves (72
+// VGPRs, two registers of cold state spilled), which traces launches of tens
+// of millions of AO rays 3-4% faster and launches of a million rays 6-7%
+// slower (more waves, each slower, lengthen the tail of a short launch).
+// The other instances stay unconstrained: the batch, surfel, counting and
+// prefetching kernels would spill 10 to 32 registers at 7 waves.
+constexpr int kOccupancyAuto = 0;
+constexpr int kOccupancyFree = 1;
+constexpr int kOccupancy7 = 7;
+constexpr long long kOccupancy7MinRays = 1ll << 22;  // automatic choice: launches this large
+
 #ifndef CWBVH_TRACE_ATTR
-#define CWBVH_TRACE_ATTR
+#define CWBVH_TRACE_ATTR __attribute__((amdgpu_waves_per_eu(W7 ? 7 : 1)))
 #endif
 
-template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer>
+template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer, bool W7 = false>
 __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhParams P) {
+    static_assert(!W7 || (LOOP == kLoopInterleave && !MULTI && !STATS && !PREFETCH && SRC == kRaysFromBuffer),
+                  "the 7-wave build is made for the plain interleaved kernel only");
     extern __shared__ unsigned long long lds_stack_raw[];
     CWD_LDS unsigned long long* lds_stack = lds_ptr(lds_stack_raw);
     unsigned long long spill[kStackCapacity];
@@ -713,11 +727,11 @@ int resident_blocks(K kernel, int block, size_t lds) {
     return nb;
 }
 
-template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer>
+template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer, bool W7 = false>
 int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t stream) {
     const int block = cfg.block_size;
     const size_t lds = (size_t)block * (size_t)cfg.lds_depth * sizeof(unsigned long long);
-    auto kernel = cwbvh_trace<ANY, STATS, MULTI, PREFETCH, LOOP, SRC>;
+    auto kernel = cwbvh_trace<ANY, STATS, MULTI, PREFETCH, LOOP, SRC, W7>;
     long long grid = cfg.grid_blocks;
     if (!MULTI && !p.persistent) {
         grid = ((long long)p.n + block - 1) / block;
@@ -751,7 +765,13 @@ int32_t launch_bvh2_t(Bvh2Params& p, const cwbvh_trace_config& cfg, hipStream_t 
 template <bool ANY, bool STATS, bool MULTI, bool PREFETCH>
 int32_t dispatch_loop(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t st) {
     switch (cfg.loop_mode) {
-        case kLoopInterleave: return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopInterleave>(p, cfg, st);
+        case kLoopInterleave:
+            if constexpr (!STATS && !MULTI && !PREFETCH) {
+                const bool w7 = cfg.occupancy == kOccupancy7 ||
+                                (cfg.occupancy == kOccupancyAuto && (long long)p.n >= kOccupancy7MinRays);
+                if (w7) return launch_cwbvh_t<ANY, false, false, false, kLoopInterleave, kRaysFromBuffer, true>(p, cfg, st);
+            }
+            return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopInterleave>(p, cfg, st);
         case kLoopSpeculate: return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopSpeculate>(p, cfg, st);
         default: return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopGroup>(p, cfg, st);
     }
@@ -817,7 +837,8 @@ int32_t validate_config(int32_t kind, const cwbvh_trace_config& c) {
             c.persistent < 0 || c.persistent > 2 ||
             c.refill_idle < 0 || c.refill_idle > 64 || c.postpone_pct < 0 || c.postpone_pct > 100 ||
             c.prefetch < 0 || c.prefetch > 1 || c.work_slices < 1 || c.work_slices > kWorkSlices ||
-            c.loop_mode < kLoopGroup || c.loop_mode > kLoopSpeculate) {
+            c.loop_mode < kLoopGroup || c.loop_mode > kLoopSpeculate ||
+            !(c.occupancy == kOccupancyAuto || c.occupancy == kOccupancyFree || c.occupancy == kOccupancy7)) {
             set_error("invalid CWBVH trace configuration");
             return CWBVH_ERR_INVALID_ARGUMENT;
         }

@@ -85,6 +85,9 @@ CWBVH_VARIANTS = [
     dict(loop_mode=0, persistent=0, block_size=64, lds_depth=4),
     dict(loop_mode=0, prefetch=1, refill_idle=8, refill_budget=32, fetch_chunk=33),
     dict(loop_mode=1),
+    dict(loop_mode=1, occupancy=7),
+    dict(loop_mode=1, occupancy=7, block_size=256, lds_depth=16, fetch_chunk=64),
+    dict(loop_mode=1, occupancy=1),
     dict(loop_mode=1, persistent=0, block_size=64),
     dict(loop_mode=1, prefetch=1, lds_depth=3, fetch_chunk=5),
     dict(loop_mode=2),
@@ -305,7 +308,8 @@ def test_large_scene_uploads_128b_nodes(cw):
 def test_invalid_loop_configs_rejected(cw, c0):
     wl, lays = c0
     sc = cw.Scene(lays["cwbvh"])
-    for kw in (dict(loop_mode=3), dict(loop_mode=-1), dict(loop_mode=1, postpone_pct=20)):
+    for kw in (dict(loop_mode=3), dict(loop_mode=-1), dict(loop_mode=1, postpone_pct=20), dict(occupancy=6),
+               dict(occupancy=-1)):
         cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw)
         with pytest.raises(cw.CwbvhError) as e:
             sc.trace_numpy(wl.rays[:64], 0, cfg)

@@ -54,6 +54,7 @@ def test_trace_config_defaults(cw):
     assert c.block_size in (64, 128, 256) and 1 <= c.lds_depth <= cw.STACK_CAPACITY
     assert c.fetch_chunk >= 0 and c.refill_budget >= 1 and c.persistent in (0, 1, 2)
     assert c.loop_mode in (0, 1, 2)
+    assert c.occupancy == 0  # automatic
     b = cw.default_trace_config(cw.KIND_BVH2)
     assert b.block_size in (64, 128, 256)
     with pytest.raises(TypeError):
