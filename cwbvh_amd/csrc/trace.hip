@@ -142,13 +142,15 @@ constexpr int kLoopInterleave = 1;
 constexpr int kLoopSpeculate = 2;
 
 // Register budget (cwbvh_trace_config.occupancy).  The plain interleaved
-// kernel (one scene, rays from a buffer, no counters, no prefetch) takes 75
-// VGPRs unconstrained: 6 waves per SIMD.  It is also built for 7 waves (72
-// VGPRs, two registers of cold state spilled), which traces launches of tens
-// of millions of AO rays 3-4% faster and launches of a million rays 6-7%
-// slower (more waves, each slower, lengthen the tail of a short launch).
-// The other instances stay unconstrained: the batch, surfel, counting and
-// prefetching kernels would spill 10 to 32 registers at 7 waves.
+// kernel (one scene, no counters, no prefetch) takes 75 VGPRs unconstrained
+// when it reads rays from a buffer (6 waves per SIMD), 91 when it makes them
+// from surfels (5 waves).  Both are also built for 7 waves (72 VGPRs; 2 and
+// 10 registers of cold state spilled), which traces launches of tens of
+// millions of AO rays 3-5% faster and launches of a million rays 6-7% slower
+// (more waves, each slower, lengthen the tail of a short launch).  The other
+// instances stay unconstrained: the batch, counting and prefetching kernels
+// spill 10 to 32 registers at 7 waves, and the grouped schedule serves scenes
+// that stream from HBM, where spills and the extra wave cost more.
 constexpr int kOccupancyAuto = 0;
 constexpr int kOccupancyFree = 1;
 constexpr int kOccupancy7 = 7;
@@ -160,8 +162,8 @@ constexpr long long kOccupancy7MinRays = 1ll << 22;  // automatic choice: launch
 
 template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer, bool W7 = false>
 __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhParams P) {
-    static_assert(!W7 || (LOOP == kLoopInterleave && !MULTI && !STATS && !PREFETCH && SRC == kRaysFromBuffer),
-                  "the 7-wave build is made for the plain interleaved kernel only");
+    static_assert(!W7 || (LOOP == kLoopInterleave && !MULTI && !STATS && !PREFETCH),
+                  "the 7-wave build is made for the plain interleaved kernels only");
     extern __shared__ unsigned long long lds_stack_raw[];
     CWD_LDS unsigned long long* lds_stack = lds_ptr(lds_stack_raw);
     unsigned long long spill[kStackCapacity];
@@ -793,7 +795,13 @@ int32_t dispatch_cwbvh(CwbvhParams& p, uint32_t flags, const cwbvh_trace_config&
 template <bool ANY, bool STATS>
 int32_t dispatch_surfels(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t st) {
     switch (cfg.loop_mode) {
-        case kLoopInterleave: return launch_cwbvh_t<ANY, STATS, false, false, kLoopInterleave, kRaysFromSurfels>(p, cfg, st);
+        case kLoopInterleave:
+            if constexpr (!STATS) {
+                const bool w7 = cfg.occupancy == kOccupancy7 ||
+                                (cfg.occupancy == kOccupancyAuto && (long long)p.n >= kOccupancy7MinRays);
+                if (w7) return launch_cwbvh_t<ANY, false, false, false, kLoopInterleave, kRaysFromSurfels, true>(p, cfg, st);
+            }
+            return launch_cwbvh_t<ANY, STATS, false, false, kLoopInterleave, kRaysFromSurfels>(p, cfg, st);
         case kLoopSpeculate: return launch_cwbvh_t<ANY, STATS, false, false, kLoopSpeculate, kRaysFromSurfels>(p, cfg, st);
         default: return launch_cwbvh_t<ANY, STATS, false, false, kLoopGroup, kRaysFromSurfels>(p, cfg, st);
     }

@@ -340,9 +340,10 @@ typedef struct cwbvh_trace_config {
                                 per iteration (reference order), 1 = node visit or one triangle test
                                 per iteration, 2 = as 1, and lanes holding triangles also visit a
                                 node, parking its triangle group on the stack */
-    int32_t occupancy;       /* register budget of the loop_mode 1 kernel (no prefetch, no stats):
-                                0 = automatic (7 waves per SIMD for launches of 2^22 rays or more),
-                                1 = unconstrained (6 waves), 7 = 7 waves per SIMD (72 VGPRs) */
+    int32_t occupancy;       /* register budget of the single-scene loop_mode 1 kernels (no prefetch,
+                                no stats; rays from a buffer or from surfels): 0 = automatic
+                                (7 waves per SIMD for launches of 2^22 rays or more), 1 = unconstrained,
+                                7 = 7 waves per SIMD (72 VGPRs) */
 } cwbvh_trace_config;
 
 int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);

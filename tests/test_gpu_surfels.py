@@ -52,7 +52,8 @@ def test_device_generator_matches_host(cw, surfels):
 
 
 @pytest.mark.parametrize("kw", [dict(), dict(loop_mode=0), dict(loop_mode=2, postpone_pct=30),
-                                dict(persistent=0), dict(persistent=1, fetch_chunk=7, block_size=64)])
+                                dict(persistent=0), dict(persistent=1, fetch_chunk=7, block_size=64),
+                                dict(occupancy=7), dict(occupancy=7, persistent=0, block_size=256)])
 def test_fused_trace_equals_trace_of_generated_rays(cw, surfels, kw):
     mesh, cwb, pos, nrm, rad, diag = surfels
     k = 16
