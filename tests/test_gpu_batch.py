@@ -80,7 +80,11 @@ def test_batch_matches_oracle(cw, batch_set, morton, any_hit):
                                 dict(block_size=256, lds_depth=32, fetch_chunk=256, prefetch=1),
                                 dict(postpone_pct=0, refill_idle=0, refill_budget=1, grid_blocks=5),
                                 dict(loop_mode=1, block_size=64, lds_depth=2, fetch_chunk=3),
-                                dict(loop_mode=2, block_size=128, lds_depth=8, fetch_chunk=64, prefetch=1)])
+                                dict(loop_mode=2, block_size=128, lds_depth=8, fetch_chunk=64, prefetch=1),
+                                # the autotuner's usual picks on many-scene batches
+                                dict(loop_mode=2, block_size=256, lds_depth=8),
+                                dict(loop_mode=2, block_size=64, lds_depth=8),
+                                dict(loop_mode=1, block_size=64, lds_depth=8)])
 def test_batch_configs(cw, batch_set, kw):
     _, items, refs = batch_set
     bufs, ditems = device_items(cw, items)
