@@ -73,9 +73,22 @@ def cpu_baseline(layout, rays, target_s: float, threads: int) -> dict:
     t0 = time.perf_counter()
     oracle.trace_cwbvh(nodes, layout.node_stride, woop, rays[:n], threads=threads)
     dt = time.perf_counter() - t0
+    # one host thread on a small sample, for the per-core figure
+    n1 = min(len(rays), 65536)
+    t1 = time.perf_counter()
+    oracle.trace_cwbvh(nodes, layout.node_stride, woop, rays[:n1], threads=1)
+    dt1 = time.perf_counter() - t1
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(n / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"first {n} AO rays of the workload, CWBVH closest hit, scalar C restatement "
-                      f"(oracle/oracle.c) on {threads} host threads, {dt:.1f} s wall"}
+                      f"(oracle/oracle.c) on {threads} host threads, {dt:.1f} s wall",
+            "single_thread_mrays_s": round(n1 / dt1 / 1e6, 3), "host_cpu": model,
+            "host_threads": os.cpu_count()}
 
 
 def bench_c2(args, cw, W, dev) -> int:
