@@ -141,11 +141,13 @@ def hit_ids(hits):
 def compare_hits(gpu, ref, woop, rays, prim_idx=None, t_rel=1e-5, any_hit=False):
     """Classify GPU hits against oracle hits.
 
-    Returns dict(total, exact, ties, mismatches=[indices]).  A closest-hit
-    mismatch counts as a grazing tie when the GPU's triangle is itself a valid
-    hit of the ray (oracle Woop test) whose t is within t_rel of the oracle's.
-    For any-hit only the occluded/not-occluded outcome is compared, plus the
-    validity of the reported triangle.
+    Returns dict(total, exact, ties, mismatches=[indices], log=[...]).  A
+    closest-hit mismatch counts as a grazing tie when the GPU's triangle is
+    itself a valid hit of the ray (oracle Woop test) whose t is within t_rel of
+    the oracle's.  For any-hit only the occluded/not-occluded outcome is
+    compared, plus the validity of the reported triangle.  `log` holds one
+    entry per ray that is not exact (ties included): its index, the ray, both
+    hits and the kind of disagreement.
     """
     gpu = _f32(gpu, 4)
     ref = _f32(ref, 4)
@@ -157,11 +159,19 @@ def compare_hits(gpu, ref, woop, rays, prim_idx=None, t_rel=1e-5, any_hit=False)
         rp = np.where(rid >= 0, pi[np.clip(rid, 0, None)], -1)
     else:
         gp, rp = gid, rid
-    out = {"total": len(ref), "exact": 0, "ties": 0, "mismatches": []}
+    out = {"total": len(ref), "exact": 0, "ties": 0, "mismatches": [], "log": []}
+
+    def log(i, kind):
+        out["log"].append({"ray_index": int(i), "ray": rays[i].tolist(), "gpu_hit": gpu[i].tolist(),
+                           "gpu_prim": int(gp[i]), "oracle_hit": ref[i].tolist(), "oracle_prim": int(rp[i]),
+                           "kind": kind})
+
     if any_hit:
         same = (gp >= 0) == (rp >= 0)
         out["exact"] = int(same.sum())
-        out["mismatches"] = list(np.nonzero(~same)[0])
+        out["mismatches"] = [int(i) for i in np.nonzero(~same)[0]]
+        for i in out["mismatches"]:
+            log(i, "occlusion differs")
         return out
     same = (gp == rp) & ((rp < 0) | (np.abs(gpu[:, 0] - ref[:, 0]) <= t_rel * np.abs(ref[:, 0])))
     out["exact"] = int(same.sum())
@@ -170,6 +180,8 @@ def compare_hits(gpu, ref, woop, rays, prim_idx=None, t_rel=1e-5, any_hit=False)
             ok, t, _, _ = woop_test(woop[gid[i]:gid[i] + 3], rays[i])
             if ok and abs(t - ref[i, 0]) <= t_rel * abs(ref[i, 0]):
                 out["ties"] += 1
+                log(i, "grazing tie")
                 continue
         out["mismatches"].append(int(i))
+        log(i, "mismatch")
     return out

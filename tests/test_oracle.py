@@ -206,3 +206,31 @@ def test_oracle_counters_and_threads_deterministic(cw, orc):
     h8, c8 = orc.trace_cwbvh(cwb.nodes, 5, cwb.woop, wl.rays, threads=8, counters=True)
     assert np.array_equal(h1.view(np.uint32), h8.view(np.uint32))
     assert np.array_equal(c1, c8) and c1[0] >= len(wl.rays)
+
+
+def test_compare_hits_logs_every_disagreement(cw, orc):
+    """Planted disagreements (on the CPU, in a copy of the oracle's hits) are
+    each logged with their ray and both hits."""
+    mesh = cw.Mesh.heightfield(16, 8, 4, 0.1, seed=5)
+    _, cwb = cw.build_layouts(mesh, bvh2=False)
+    diag = mesh.diagonal()
+    rays = cw.rays_ao_surface(mesh, 300, 4, 1e-4 * diag, 0.0, 0.2 * diag, seed=6)
+    ref = orc.trace_cwbvh(cwb.nodes, 5, cwb.woop, rays)
+    ids = orc.hit_ids(ref)
+    hit, miss = np.nonzero(ids >= 0)[0], np.nonzero(ids < 0)[0]
+    assert len(hit) > 3 and len(miss) > 3
+    got = ref.copy()
+    got[miss[0], 1] = ref[hit[0], 1]           # a miss reported as a hit
+    got[hit[1], 1] = np.array([-1], np.int32).view(np.float32)[0]   # a hit reported as a miss
+    clean = orc.compare_hits(ref, ref, cwb.woop, rays, cwb.prim_idx)
+    assert clean["exact"] == len(rays) and clean["log"] == []
+    c = orc.compare_hits(got, ref, cwb.woop, rays, cwb.prim_idx)
+    assert sorted(c["mismatches"]) == sorted([int(miss[0]), int(hit[1])])
+    logged = {e["ray_index"]: e for e in c["log"]}
+    assert set(logged) == {int(miss[0]), int(hit[1])}
+    e = logged[int(miss[0])]
+    assert e["kind"] == "mismatch" and np.array_equal(np.float32(e["ray"]), rays[miss[0]])
+    assert e["oracle_prim"] == -1 and e["gpu_prim"] >= 0
+    a = orc.compare_hits(got, ref, cwb.woop, rays, cwb.prim_idx, any_hit=True)
+    assert {x["ray_index"] for x in a["log"]} == {int(miss[0]), int(hit[1])}
+    assert all(x["kind"] == "occlusion differs" for x in a["log"])
