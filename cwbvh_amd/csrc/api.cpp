@@ -6,6 +6,8 @@
 #include <cstdio>
 #include <map>
 #include <mutex>
+#include <sched.h>
+
 #include <thread>
 
 #include "internal.h"
@@ -25,10 +27,17 @@ int32_t hip_fail(hipError_t e, const char* what) {
                                     : (e == hipErrorNoDevice ? CWBVH_ERR_NO_DEVICE : CWBVH_ERR_HIP);
 }
 
+// Host worker threads by default: the CPUs this process may run on (its
+// affinity mask, which a shared machine narrows to the job's share), at most
+// 64; hardware_concurrency() when the mask cannot be read.
 int num_threads_or_default(int requested) {
     if (requested > 0) return requested;
-    unsigned h = std::thread::hardware_concurrency();
-    return h ? (int)h : 1;
+    int n = 0;
+    cpu_set_t set;
+    CPU_ZERO(&set);
+    if (sched_getaffinity(0, sizeof(set), &set) == 0) n = CPU_COUNT(&set);
+    if (n <= 0) n = (int)std::thread::hardware_concurrency();
+    return std::max(1, std::min(n, 64));
 }
 
 int device_compute_units() {

@@ -247,11 +247,23 @@ struct Builder {
 
     // Runs bin_range(lo, hi, out) over [b, e) into `bins_out`, on all threads
     // for large ranges (partial bins merged in thread order).
+    // Helper threads of the parallel binning, over all tasks at once: a task
+    // takes them only while the total stays within max_threads, so nested
+    // parallelism never multiplies the thread count.
+    std::atomic<int> bin_helpers{0};
+
     template <class B, class F, class M>
     void bin_all(int32_t b, int32_t e, size_t count, std::vector<B>& bins_out, F&& bin_range, M&& merge) {
         const int32_t n = e - b;
+        int t = 1;
         if (n >= kParallelBinMin && max_threads > 1) {
-            int t = max_threads;
+            t = std::min(max_threads, 8);
+            if (bin_helpers.fetch_add(t) + t > max_threads) {
+                bin_helpers.fetch_sub(t);
+                t = 1;
+            }
+        }
+        if (t > 1) {
             std::vector<std::vector<B>> part(t, std::vector<B>(count));
             std::vector<std::thread> pool;
             int32_t chunk = (n + t - 1) / t;
@@ -261,6 +273,7 @@ struct Builder {
                 pool.emplace_back([&, lo, hi, i] { bin_range(lo, hi, part[i]); });
             }
             for (auto& th : pool) th.join();
+            bin_helpers.fetch_sub(t);
             for (auto& pt : part)
                 for (size_t k = 0; k < count; ++k) merge(bins_out[k], pt[k]);
         } else {
