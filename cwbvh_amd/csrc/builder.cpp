@@ -3,8 +3,12 @@
 // Replaces the reference's Embree-based builder (src/EmbreeBVHBuilder.cpp:303-478,
 // which needs Embree3 + TBB) and its converters (src/GPUBVHConverter.cpp:10-403).
 //
-// One binned-SAH binary tree with single-triangle leaves is built per mesh.
-// Two products are derived from it:
+// One binned-SAH binary tree with single-reference leaves is built per mesh,
+// on a work queue of subtrees; with spatial splits (split_alpha > 0) a
+// reference can be the part of a triangle on one side of split planes, the
+// role of the reference's SplitPrimitiveTriangle (EmbreeBVHBuilder.cpp:110-247).
+// Two products are derived from it, each written on all threads in the byte
+// order of the serial depth-first pass that defines it:
 //   * BVH2: leaves merged bottom-up wherever the SAH leaf cost is lower
 //     (bounded by bvh2_max_leaf), written in the reference BVH2 layout
 //     (GPUBVHConverter.cpp:10-121): 4 float4 per node, child ints >= 0 are
@@ -34,7 +38,6 @@
 #include <condition_variable>
 #include <memory>
 #include <mutex>
-#include <numeric>
 #include <thread>
 
 #include "common.h"
