@@ -53,11 +53,16 @@ int device_compute_units() {
     return cus;
 }
 
+// Keyed by (current device, stream): the null stream names a different queue
+// on each device, and each device needs counters in its own memory.
 int32_t stream_counters(hipStream_t stream, unsigned** out) {
     static std::mutex mu;
-    static std::map<hipStream_t, unsigned*> table;
+    static std::map<std::pair<int, hipStream_t>, unsigned*> table;
+    int dev = 0;
+    CW_HIP(hipGetDevice(&dev));
+    const std::pair<int, hipStream_t> key(dev, stream);
     std::lock_guard<std::mutex> g(mu);
-    auto it = table.find(stream);
+    auto it = table.find(key);
     if (it != table.end()) {
         *out = it->second;
         return CWBVH_OK;
@@ -65,7 +70,7 @@ int32_t stream_counters(hipStream_t stream, unsigned** out) {
     unsigned* p = nullptr;
     CW_HIP(hipMalloc(&p, kWorkWords * sizeof(unsigned)));
     CW_HIP(hipMemset(p, 0, kWorkWords * sizeof(unsigned)));
-    table[stream] = p;
+    table[key] = p;
     *out = p;
     return CWBVH_OK;
 }
@@ -320,6 +325,8 @@ int32_t cwbvh_scene_create_ex(const cwbvh_layout* L, void* stream, int32_t strid
         e = hipMemcpyAsync(dp, L->prim_idx.data(), L->prim_idx.size() * sizeof(int32_t), hipMemcpyHostToDevice, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
+        // copies queued before the failing call read host vectors freed on return
+        (void)hipStreamSynchronize(st);
         if (dn) hipFree(dn);
         if (dw) hipFree(dw);
         if (dp) hipFree(dp);

@@ -136,6 +136,7 @@ struct cwbvh_server {
 
     int32_t launch(Slot& sl, const std::vector<Request*>& batch);
     void complete(Slot& sl);
+    void fail_pending();
     void loop();
 };
 
@@ -215,6 +216,20 @@ void cwbvh_server::complete(Slot& sl) {
     cv_done.notify_all();
 }
 
+// After a device failure no further batch is launched: every request still
+// waiting for the arena ends with the failure status, so waiters, drain and
+// destroy return.  Called with `mu` held.
+void cwbvh_server::fail_pending() {
+    const double t = now_s();
+    for (Request* r : pending) {
+        r->status = worker_status;
+        r->done = true;
+        r->t_done = t;
+        --outstanding;
+    }
+    pending.clear();
+}
+
 void cwbvh_server::loop() {
     std::vector<Request*> batch;
     while (true) {
@@ -226,16 +241,20 @@ void cwbvh_server::loop() {
                 complete(sl);
                 progress = true;
             } else if (q != hipErrorNotReady) {
+                const double t = now_s();
                 std::lock_guard<std::mutex> g(mu);
                 worker_status = hip_fail(q, "serving batch");
                 worker_error = get_error();
                 for (Request* r : sl.reqs) {
                     r->status = worker_status;
                     r->done = true;
+                    r->t_done = t;
                     --outstanding;
                 }
                 sl.reqs.clear();
                 sl.busy = false;
+                if (--inflight == 0) stats.busy_s += t - busy_since;
+                fail_pending();
                 cv_done.notify_all();
             }
         }
@@ -279,16 +298,24 @@ void cwbvh_server::loop() {
         if (!batch.empty()) {
             int32_t rc = launch(*free_slot, batch);
             if (rc) {
+                const std::string why = get_error();
+                // copies queued before the failing call may still read the
+                // callers' rays or write their hits: let them finish before
+                // the requests are reported
+                (void)hipStreamSynchronize(free_slot->stream);
+                const double t = now_s();
                 std::lock_guard<std::mutex> g(mu);
                 worker_status = rc;
-                worker_error = get_error();
+                worker_error = why;
                 for (Request* r : batch) {
                     r->status = rc;
                     r->done = true;
+                    r->t_done = t;
                     arena.release(r->off, align_up(r->n * (int64_t)sizeof(cwbvh_ray)) + r->n * (int64_t)sizeof(cwbvh_hit));
                     --outstanding;
                 }
-                --inflight;
+                if (--inflight == 0) stats.busy_s += t - busy_since;
+                fail_pending();
                 cv_done.notify_all();
             }
         }
