@@ -465,7 +465,10 @@ int32_t cwbvh_server_submit(cwbvh_server* s, int32_t scene, const cwbvh_ray* hos
  * submission on the host clock. */
 int32_t cwbvh_server_wait(cwbvh_server* s, uint64_t request_id, double timeout_s,
                           double* latency_ms);
-/* Wait until every submitted request has completed. */
+/* Wait until every submitted request has completed.  After a device error
+ * the server launches nothing more: every queued or in-flight request
+ * completes with that status (wait and drain return it) and later
+ * submissions are refused with it. */
 int32_t cwbvh_server_drain(cwbvh_server* s);
 
 typedef struct cwbvh_server_stats {
@@ -485,7 +488,9 @@ void cwbvh_server_destroy(cwbvh_server* s);
  * request log-uniform in [min_rays, max_rays], scene uniform over the
  * server's scenes.  Request k's rays are a window of ray_pools[scene]
  * (pinned host arrays of pool_sizes[scene] rays); hits land in hit_pool
- * (pinned, hit_pool_size hits, used as a ring).  latencies_ms (nullable,
+ * (pinned, hit_pool_size hits, used as a ring, so requests in flight together
+ * may write the same slots: the generator measures load, its hits are not
+ * results).  latencies_ms (nullable,
  * num_requests doubles) receives each request's latency in arrival order. */
 typedef struct cwbvh_poisson_spec {
     double rate;
