@@ -93,6 +93,7 @@ class TraceConfig(C.Structure):
         ("persistent", C.c_int32),
         ("loop_mode", C.c_int32),
         ("occupancy", C.c_int32),
+        ("ray_order", C.c_int32),
     ]
 
     def as_dict(self) -> dict:

@@ -178,7 +178,9 @@ int32_t cwbvh_stream_create(void** stream) {
 }
 
 int32_t cwbvh_stream_destroy(void* stream) {
-    if (stream) CW_HIP(hipStreamDestroy((hipStream_t)stream));
+    if (!stream) return CWBVH_OK;
+    cw::release_stream_scratch((hipStream_t)stream);
+    CW_HIP(hipStreamDestroy((hipStream_t)stream));
     return CWBVH_OK;
 }
 
@@ -443,11 +445,19 @@ int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_confi
     int32_t rc = cwbvh_trace_config_default(scene->kind, c);
     if (rc) return rc;
     // measured on MI355X: a tree that fits the 256 MB Infinity Cache is
-    // VALU-bound and gains from the interleaved schedule (C1: 1.36x); a
-    // larger one is bound by HBM traffic, which the grouped schedule keeps
-    // lower by testing a leaf's triangles right after fetching the node
-    // (C3, 1.2 GB: 1.10x over interleaved)
-    if (scene->kind == CWBVH_KIND_CWBVH && scene->bytes > kCacheResidentBytes) c->loop_mode = 0;
+    // VALU-bound and gains from the interleaved schedule (C1: 1.36x).  A
+    // larger one is bound by HBM traffic.  Sorting a launch's rays by origin
+    // first lets neighbouring waves share the nodes they fetch (C3, 1.7 GB,
+    // 67 M shuffled rays: the 2.6 ms sort and the Morton-ordered trace
+    // together are 1.14x the trace in the given order), and sorted rays run
+    // best with the speculative schedule (C3: 1.05x over grouped).  Wrapped
+    // device arrays carry no scene bounds to quantize origins in; unsorted
+    // rays keep the grouped schedule, which tests a leaf's triangles right
+    // after fetching the node (C3 shuffled: 1.10x over interleaved).
+    if (scene->bytes > kCacheResidentBytes) {
+        if (scene->owning) c->ray_order = 1;
+        if (scene->kind == CWBVH_KIND_CWBVH) c->loop_mode = scene->owning ? 2 : 0;
+    }
     return CWBVH_OK;
 }
 
@@ -478,6 +488,14 @@ static int32_t trace_impl(const cwbvh_scene* scene, const cwbvh_ray* d_rays, con
     L.stream = (hipStream_t)stream;
     rc = stream_counters(L.stream, &L.counters);
     if (rc) return rc;
+    if (c.ray_order == 1 && d_order == nullptr && n >= kMortonMinRays) {
+        // the order lives in the stream's workspace: keep it locked until
+        // the traversal that reads it is queued behind the sort
+        auto lock = lock_stream_scratch();
+        rc = morton_order_scratch(d_rays, n, scene->lo, scene->hi, L.stream, &L.order);
+        if (rc) return rc;
+        return scene->kind == CWBVH_KIND_CWBVH ? launch_cwbvh(L) : launch_bvh2(L);
+    }
     return scene->kind == CWBVH_KIND_CWBVH ? launch_cwbvh(L) : launch_bvh2(L);
 }
 

@@ -12,6 +12,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "device_common.h"
@@ -116,6 +118,67 @@ struct cwbvh_batch {
 
 namespace {
 
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+// Per (device, stream) device workspace of the Morton pre-pass, grown on
+// demand and kept for later calls: a call on a stream reuses it once the
+// stream's earlier work is ordered before it, so only growth allocates.
+// Released by cwbvh_stream_destroy (or at process end).  Callers hold
+// g_scratch_mu from lookup until their kernels are enqueued, so a buffer is
+// never freed between a call's lookup and its launches.
+std::mutex g_scratch_mu;
+std::map<std::pair<int, hipStream_t>, std::pair<char*, size_t>> g_scratch;
+
+int32_t stream_scratch(hipStream_t st, size_t bytes, char** out) {
+    int dev = 0;
+    CW_HIP(hipGetDevice(&dev));
+    auto& slot = g_scratch[std::make_pair(dev, st)];
+    if (slot.second < bytes) {
+        if (slot.first) {
+            (void)hipFree(slot.first);   // waits for the device, so no queued call still uses it
+            slot = {nullptr, 0};
+        }
+        // 1/8 headroom so slowly growing calls do not reallocate every time
+        const size_t want = bytes + bytes / 8;
+        CW_HIP(hipMalloc(&slot.first, want));
+        slot.second = want;
+    }
+    *out = slot.first;
+    return CWBVH_OK;
+}
+
+// Origin Morton order of n rays on stream st, in the stream's workspace
+// (g_scratch_mu held by the caller): keys, sorted keys, ray indices, sort
+// temporaries, and, when d_order is null, the order itself (*in_scratch).
+int32_t morton_sort(const void* d_rays, int64_t n, const float* lo3, const float* hi3, hipStream_t st,
+                    int32_t* d_order, int32_t** in_scratch) {
+    size_t tmp_bytes = 0;
+    unsigned* null_keys = nullptr;
+    int32_t* null_vals = nullptr;
+    CW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, null_keys, null_keys, null_vals, null_vals, (int)n, 0,
+                                              30, st));
+    const size_t col = align256((size_t)n * 4);
+    const int cols = d_order ? 3 : 4;
+    const size_t need = cols * col + align256(std::max<size_t>(tmp_bytes, 16));
+    char* base = nullptr;
+    int32_t rc = stream_scratch(st, need, &base);
+    if (rc) return rc;
+    auto* k0 = reinterpret_cast<unsigned*>(base);
+    auto* k1 = reinterpret_cast<unsigned*>(base + col);
+    auto* v0 = reinterpret_cast<int32_t*>(base + 2 * col);
+    if (!d_order) {
+        d_order = reinterpret_cast<int32_t*>(base + 3 * col);
+        *in_scratch = d_order;
+    }
+    void* tmp = base + cols * col;
+    const int block = 256;
+    hipLaunchKernelGGL(morton_keys_single, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0, st,
+                       (const Ray4*)d_rays, (int)n, make_float3(lo3[0], lo3[1], lo3[2]), scale_of(lo3, hi3), k0, v0);
+    CW_HIP(hipGetLastError());
+    CW_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, v0, d_order, (int)n, 0, 30, st));
+    return CWBVH_OK;
+}
+
 int32_t batch_sort(cwbvh_batch* b, hipStream_t st) {
     if (!b->morton || b->total_rays == 0) return CWBVH_OK;
     const int block = 256;
@@ -147,6 +210,30 @@ int32_t batch_trace(cwbvh_batch* b, hipStream_t st) {
 
 using namespace cw;
 
+namespace cw {
+
+std::unique_lock<std::mutex> lock_stream_scratch() { return std::unique_lock<std::mutex>(g_scratch_mu); }
+
+int32_t morton_order_scratch(const void* d_rays, int64_t n, const float lo[3], const float hi[3], hipStream_t st,
+                             const int32_t** order) {
+    int32_t* out = nullptr;
+    int32_t rc = morton_sort(d_rays, n, lo, hi, st, nullptr, &out);
+    *order = out;
+    return rc;
+}
+
+void release_stream_scratch(hipStream_t st) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    auto it = g_scratch.find(std::make_pair(dev, st));
+    if (it == g_scratch.end()) return;
+    if (it->second.first) (void)hipFree(it->second.first);
+    g_scratch.erase(it);
+}
+
+}  // namespace cw
+
 extern "C" {
 
 int32_t cwbvh_scene_bounds(const cwbvh_scene* s, float* lo3, float* hi3) {
@@ -163,33 +250,8 @@ int32_t cwbvh_morton_order(const cwbvh_ray* d_rays, int64_t n, const float* lo3,
     CW_CHECK_ARG(n >= 0 && n <= 0x7FFFFFFF, "ray count must be in [0, 2^31)");
     CW_CHECK_ARG(n == 0 || (d_rays && d_order && lo3 && hi3), "NULL argument");
     if (n == 0) return CWBVH_OK;
-    hipStream_t st = (hipStream_t)stream;
-    unsigned *k0 = nullptr, *k1 = nullptr;
-    int32_t* v0 = nullptr;
-    void* tmp = nullptr;
-    size_t tmp_bytes = 0;
-    int32_t rc = CWBVH_OK;
-    hipError_t e = hipMalloc(&k0, n * sizeof(unsigned));
-    if (e == hipSuccess) e = hipMalloc(&k1, n * sizeof(unsigned));
-    if (e == hipSuccess) e = hipMalloc(&v0, n * sizeof(int32_t));
-    if (e == hipSuccess)
-        e = hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, k0, k1, v0, d_order, (int)n, 0, 30, st);
-    if (e == hipSuccess) e = hipMalloc(&tmp, std::max<size_t>(tmp_bytes, 16));
-    if (e == hipSuccess) {
-        const int block = 256;
-        hipLaunchKernelGGL(morton_keys_single, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0, st,
-                           (const Ray4*)d_rays, (int)n, make_float3(lo3[0], lo3[1], lo3[2]), scale_of(lo3, hi3), k0,
-                           v0);
-        e = hipGetLastError();
-    }
-    if (e == hipSuccess) e = hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, v0, d_order, (int)n, 0, 30, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) rc = hip_fail(e, "cwbvh_morton_order");
-    if (k0) (void)hipFree(k0);
-    if (k1) (void)hipFree(k1);
-    if (v0) (void)hipFree(v0);
-    if (tmp) (void)hipFree(tmp);
-    return rc;
+    auto lock = lock_stream_scratch();
+    return morton_sort(d_rays, n, lo3, hi3, (hipStream_t)stream, d_order, nullptr);
 }
 
 int32_t cwbvh_batch_create(const cwbvh_batch_item* items, int32_t count, uint32_t flags,

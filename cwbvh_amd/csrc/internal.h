@@ -55,6 +55,18 @@ constexpr int kWorkOverflow = kWorkExited + kWorkStride;
 constexpr int kWorkProfile = kWorkOverflow + kWorkStride;   // uint64[6] loop-occupancy counters
 constexpr int kWorkWords = kWorkProfile + kWorkStride;
 int32_t stream_counters(hipStream_t stream, unsigned** out);
+// Frees the Morton workspace kept for `stream` on the current device.
+void release_stream_scratch(hipStream_t stream);
+// The lock that guards every stream's workspace: hold it from a workspace
+// lookup until the kernels that use the workspace are queued.
+std::unique_lock<std::mutex> lock_stream_scratch();
+// With that lock held: origin Morton order of n rays (n >= 1) into the
+// stream's workspace; *order points into it until the stream's next use.
+int32_t morton_order_scratch(const void* d_rays, int64_t n, const float lo[3], const float hi[3],
+                             hipStream_t stream, const int32_t** order);
+// cwbvh_trace_config.ray_order = 1 sorts launches of at least this many rays
+// (below it the sort's fixed cost outweighs the gain)
+constexpr int64_t kMortonMinRays = int64_t(1) << 20;
 
 struct TraceLaunch {
     const cwbvh_scene* scene;

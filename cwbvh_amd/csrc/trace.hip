@@ -840,6 +840,10 @@ int32_t validate_config(int32_t kind, const cwbvh_trace_config& c) {
         set_error("LDS stack exceeds 160 KiB per block");
         return CWBVH_ERR_INVALID_ARGUMENT;
     }
+    if (c.ray_order < 0 || c.ray_order > 1) {
+        set_error("ray_order must be 0 or 1");
+        return CWBVH_ERR_INVALID_ARGUMENT;
+    }
     if (kind == CWBVH_KIND_CWBVH) {
         if (c.fetch_chunk < 0 || c.fetch_chunk > (1 << 20) || c.grid_blocks < 0 || c.refill_budget < 1 ||
             c.persistent < 0 || c.persistent > 2 ||

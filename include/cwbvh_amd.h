@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define CWBVH_ABI_VERSION 1
+#define CWBVH_ABI_VERSION 2
 
 /* ------------------------------------------------------------------ types */
 
@@ -344,13 +344,21 @@ typedef struct cwbvh_trace_config {
                                 no stats; rays from a buffer or from surfels): 0 = automatic
                                 (7 waves per SIMD for launches of 2^22 rays or more), 1 = unconstrained,
                                 7 = 7 waves per SIMD (72 VGPRs) */
+    int32_t ray_order;       /* 0 = trace rays in the given order; 1 = a launch of 2^20 rays or more
+                                first sorts them by origin Morton code (as cwbvh_morton_order, in the
+                                stream's workspace) and traces them in that order.  Hits still go to
+                                each ray's own slot, so results do not change.  Applies to
+                                cwbvh_trace; ignored when an order is given and by surfel traces */
 } cwbvh_trace_config;
 
 int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
 
 /* The configuration cwbvh_trace uses when cfg is NULL: the kind's default,
- * with the grouped schedule (loop_mode 0) for CWBVH scenes larger than the
- * 256 MB Infinity Cache, whose traversal is bound by HBM traffic. */
+ * except for scenes larger than the 256 MB Infinity Cache, whose traversal is
+ * bound by HBM traffic.  A scene uploaded from a host layout then gets the
+ * Morton ray pre-pass (ray_order 1) and, for CWBVH, the speculative schedule
+ * (loop_mode 2); a wrapped one (no bounds) gets the grouped schedule
+ * (loop_mode 0). */
 int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_config* cfg);
 
 /* Traverse n rays.  d_counters (nullable, CWBVH_TRACE_STATS only) receives
@@ -400,7 +408,10 @@ int32_t cwbvh_scene_bounds(const cwbvh_scene* scene, float* lo3, float* hi3);
 
 /* Morton ray order: d_order[i] = index of the i-th ray when rays are sorted by
  * the 30-bit Morton code of their origin quantized inside [lo, hi].
- * d_order holds n int32.  Temporary device memory is allocated per call. */
+ * d_order holds n int32.  Stream-ordered (the call does not wait).  The
+ * keys and sort temporaries (about 12 B per ray) live in a workspace kept per
+ * device and stream and grown on demand, so repeated calls do not allocate;
+ * cwbvh_stream_destroy releases a stream's workspace. */
 int32_t cwbvh_morton_order(const cwbvh_ray* d_rays, int64_t n, const float* lo3, const float* hi3,
                            int32_t* d_order, void* stream);
 

@@ -1,6 +1,7 @@
 """GPU tests of the multi-scene persistent batch, Morton ordering, hipGraph
 capture/replay and the autotuner: every item's hits must equal the CPU
 oracle's for that scene, bit for bit, in every form."""
+import gc
 import sys
 
 import numpy as np
@@ -152,6 +153,52 @@ def test_morton_order_single_scene(cw, orc, batch_set):
     assert np.all(np.diff(code) >= 0)
     g = sc.trace_numpy(rays, 0, order=order)
     assert np.array_equal(bits(g), bits(refs[7][False]))
+
+
+def _morton_codes(rays, lo, hi):
+    """Keys of the Morton pre-pass in float32: scale = 1024 / (hi - lo),
+    q = trunc(clamp((o - lo) * scale, 0, 1023)) per axis, x bits highest."""
+    lo, hi = np.float32(lo), np.float32(hi)
+    scale = np.float32(1024.0) / (hi - lo)
+    f = np.minimum(np.maximum((rays[:, :3] - lo) * scale, np.float32(0)), np.float32(1023))
+    q = f.astype(np.uint32).astype(np.int64)
+    code = np.zeros(len(rays), np.int64)
+    for b in range(10):
+        for axis, shift in ((0, 2), (1, 1), (2, 0)):
+            code |= ((q[:, axis] >> b) & 1) << (3 * b + shift)
+    return code
+
+
+def test_morton_order_reuses_its_workspace(cw):
+    """Calls of shrinking and growing size on one stream reuse and grow the
+    kept workspace; each order is the stable sort of the float32 keys (ties
+    keep ray order), origins outside [lo, hi] clamp; destroying the stream
+    releases its workspace and later calls on the null stream still work."""
+    rng = np.random.default_rng(21)
+    lo = np.array([-1.0, -2.0, -3.0], np.float32)
+    hi = np.array([2.0, 1.0, 0.5], np.float32)
+
+    def check(n, stream):
+        say(f"morton workspace n={n} stream={'own' if stream else 'null'}")
+        rays = np.zeros((n, 8), np.float32)
+        rays[:, :3] = rng.uniform(lo - 0.1, hi + 0.1, size=(n, 3)).astype(np.float32)
+        rays[: n // 4, :3] = np.round(rays[: n // 4, :3], 1)   # repeated keys
+        rays[:, 4], rays[:, 7] = 1.0, np.inf
+        dr = cw.DeviceBuffer.from_numpy(rays)
+        do = cw.DeviceBuffer(4 * n)
+        cw.morton_order(dr.ptr, n, lo, hi, do.ptr, stream)
+        if stream is not None:
+            stream.synchronize()
+        order = do.download((n,), np.int32)
+        assert np.array_equal(order, np.argsort(_morton_codes(rays, lo, hi), kind="stable"))
+
+    st = cw.Stream()
+    for n in (50_000, 1_000, 300_000, 7):
+        check(n, st)
+    del st
+    gc.collect()
+    check(20_000, None)
+    check(60_000, None)
 
 
 def test_autotune_small(cw, batch_set):

@@ -269,10 +269,12 @@ def test_wrapped_scene_and_scene_default_config(cw, c0, c0_ref):
             assert_bitwise(sc.trace_numpy(wl.rays, int(any_hit)), c0_ref[(k, any_hit)], f"wrapped {k} any={any_hit}")
     small = cw.Scene(lays["cwbvh"]).default_config()
     assert small.loop_mode == cw.default_trace_config(cw.KIND_CWBVH).loop_mode == 1
-    # a CWBVH tree larger than the 256 MB Infinity Cache: the grouped schedule
+    assert small.ray_order == 0
+    # a wrapped CWBVH tree larger than the 256 MB Infinity Cache: no bounds to
+    # sort rays in, so rays in the given order with the grouped schedule
     big_nodes = cw.DeviceBuffer(300 << 20)
     big = cw.Scene.wrap(cw.KIND_CWBVH, 5, big_nodes.ptr, big_nodes.ptr, None, 8, keep=(big_nodes,))
-    assert big.default_config().loop_mode == 0
+    assert big.default_config().loop_mode == 0 and big.default_config().ray_order == 0
 
 
 def test_scene_node_stride_modes(cw, c0, c0_ref):
@@ -294,31 +296,47 @@ def test_scene_node_stride_modes(cw, c0, c0_ref):
 
 def test_large_scene_uploads_128b_nodes(cw):
     """A CWBVH scene larger than the Infinity Cache (here 5 M triangles,
-    about 320 MB of nodes and Woop rows) gets 128 B nodes by default, with
-    the hits of the 80 B layout."""
+    about 320 MB of nodes and Woop rows) gets 128 B nodes by default, and its
+    default configuration sorts launches of 2^20 rays or more by origin
+    Morton code first; hits equal those of the 80 B layout traced in the
+    given order.  The BVH2 scene of the same mesh sorts by default too."""
     say("building 5 M-triangle soup")
     mesh = cw.Mesh.cluster_soup(5_000_000, 1024, 0.005, 0.05, float(np.log(1e-3)), 0.5, seed=41)
-    _, cwb = cw.build_layouts(mesh, bvh2=False)
+    bvh2, cwb = cw.build_layouts(mesh)
     st = cwb.stats()
     assert 16 * (st["nodes_f4"] + st["woop_f4"]) > (256 << 20)
     auto = cw.Scene(cwb)
     keep = cw.Scene(cwb, stride_mode=cw.SCENE_STRIDE_KEEP)
     assert auto.node_stride == 8 and keep.node_stride == 5
+    assert auto.default_config().loop_mode == 2 and auto.default_config().ray_order == 1
+    as_given = keep.default_config()
+    as_given.ray_order = 0
     diag = mesh.diagonal()
-    rays = cw.rays_sphere_from_surface(mesh, 200_000, 1e-4 * diag, 0.0, np.inf, seed=42)
+    rays = cw.rays_sphere_from_surface(mesh, (1 << 20) + 4321, 1e-4 * diag, 0.0, np.inf, seed=42)
     for any_hit in (False, True):
         say(f"large scene any={any_hit}")
         a = auto.trace_numpy(rays, int(any_hit))
-        b = keep.trace_numpy(rays, int(any_hit))
-        assert_bitwise(a, b, f"128 B vs 80 B nodes any={any_hit}")
-    assert auto.default_config().loop_mode == 0
+        b = keep.trace_numpy(rays, int(any_hit), as_given)
+        assert_bitwise(a, b, f"128 B nodes, sorted vs 80 B nodes, given order, any={any_hit}")
+    s2 = cw.Scene(bvh2)
+    assert s2.default_config().ray_order == 1
+    b2_given = s2.default_config()
+    b2_given.ray_order = 0
+    for any_hit in (False, True):
+        say(f"large BVH2 scene any={any_hit}")
+        a = s2.trace_numpy(rays, int(any_hit))
+        b = s2.trace_numpy(rays, int(any_hit), b2_given)
+        assert_bitwise(a, b, f"BVH2 sorted vs given order any={any_hit}")
+        if not any_hit:
+            # the two kinds agree on every hit distance
+            assert np.array_equal(a[:, 0].view(np.uint32), auto.trace_numpy(rays, 0)[:, 0].view(np.uint32))
 
 
 def test_invalid_loop_configs_rejected(cw, c0):
     wl, lays = c0
     sc = cw.Scene(lays["cwbvh"])
     for kw in (dict(loop_mode=3), dict(loop_mode=-1), dict(loop_mode=1, postpone_pct=20), dict(occupancy=6),
-               dict(occupancy=-1)):
+               dict(occupancy=-1), dict(ray_order=2), dict(ray_order=-1)):
         cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw)
         with pytest.raises(cw.CwbvhError) as e:
             sc.trace_numpy(wl.rays[:64], 0, cfg)

@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol(cw):
 
 
 def test_abi_version_and_status_strings(cw):
-    assert cw.lib.cwbvh_abi_version() == 1
+    assert cw.lib.cwbvh_abi_version() == 2
     assert cw.lib.cwbvh_status_string(0) == b"ok"
     assert cw.lib.cwbvh_status_string(4).startswith(b"tree deeper")
 
@@ -55,6 +55,7 @@ def test_trace_config_defaults(cw):
     assert c.fetch_chunk >= 0 and c.refill_budget >= 1 and c.persistent in (0, 1, 2)
     assert c.loop_mode in (0, 1, 2)
     assert c.occupancy == 0  # automatic
+    assert c.ray_order == 0  # rays in the given order
     b = cw.default_trace_config(cw.KIND_BVH2)
     assert b.block_size in (64, 128, 256)
     with pytest.raises(TypeError):
