@@ -450,13 +450,19 @@ int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_confi
     // first lets neighbouring waves share the nodes they fetch (C3, 1.7 GB,
     // 67 M shuffled rays: the 2.6 ms sort and the Morton-ordered trace
     // together are 1.14x the trace in the given order), and sorted rays run
-    // best with the speculative schedule (C3: 1.05x over grouped).  Wrapped
+    // best with the speculative schedule (C3: 1.05x over grouped) and node
+    // prefetch, whose loads of the next node overlap the triangle tests
+    // (C3, 16.8 M rays: 1.07x; deeper LDS stacks and other refill budgets
+    // add nothing on top).  Wrapped
     // device arrays carry no scene bounds to quantize origins in; unsorted
     // rays keep the grouped schedule, which tests a leaf's triangles right
     // after fetching the node (C3 shuffled: 1.10x over interleaved).
     if (scene->bytes > kCacheResidentBytes) {
         if (scene->owning) c->ray_order = 1;
-        if (scene->kind == CWBVH_KIND_CWBVH) c->loop_mode = scene->owning ? 2 : 0;
+        if (scene->kind == CWBVH_KIND_CWBVH) {
+            c->loop_mode = scene->owning ? 2 : 0;
+            c->prefetch = scene->owning ? 1 : 0;
+        }
     }
     return CWBVH_OK;
 }

@@ -357,8 +357,8 @@ int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
  * except for scenes larger than the 256 MB Infinity Cache, whose traversal is
  * bound by HBM traffic.  A scene uploaded from a host layout then gets the
  * Morton ray pre-pass (ray_order 1) and, for CWBVH, the speculative schedule
- * (loop_mode 2); a wrapped one (no bounds) gets the grouped schedule
- * (loop_mode 0). */
+ * with node prefetch (loop_mode 2, prefetch 1); a wrapped one (no bounds)
+ * gets the grouped schedule (loop_mode 0). */
 int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_config* cfg);
 
 /* Traverse n rays.  d_counters (nullable, CWBVH_TRACE_STATS only) receives

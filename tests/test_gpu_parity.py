@@ -308,7 +308,8 @@ def test_large_scene_uploads_128b_nodes(cw):
     auto = cw.Scene(cwb)
     keep = cw.Scene(cwb, stride_mode=cw.SCENE_STRIDE_KEEP)
     assert auto.node_stride == 8 and keep.node_stride == 5
-    assert auto.default_config().loop_mode == 2 and auto.default_config().ray_order == 1
+    d = auto.default_config()
+    assert d.loop_mode == 2 and d.prefetch == 1 and d.ray_order == 1
     as_given = keep.default_config()
     as_given.ray_order = 0
     diag = mesh.diagonal()
