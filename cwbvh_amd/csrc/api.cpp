@@ -227,15 +227,16 @@ int32_t cwbvh_event_elapsed_ms(void* start, void* stop, float* ms) {
 
 // ------------------------------------------------------------ scenes
 
-// Scene bounds from the root node: decoded child boxes (CWBVH) or the two
-// child boxes (BVH2); empty or inverted boxes are skipped.
-static void layout_bounds(const cwbvh_layout& L, float lo[3], float hi[3]) {
+// Scene bounds from the root node (its first 5 float4 for CWBVH, 4 for
+// BVH2): decoded child boxes (CWBVH) or the two child boxes (BVH2); empty or
+// inverted boxes are skipped.  False (and the unit cube) when none is left.
+static bool root_bounds(int32_t kind, const cwbvh_float4* root, float lo[3], float hi[3]) {
     float a[3] = {INFINITY, INFINITY, INFINITY}, b[3] = {-INFINITY, -INFINITY, -INFINITY};
-    if (L.kind == CWBVH_KIND_CWBVH && L.nodes.size() >= 5) {
-        const uint8_t* n0 = reinterpret_cast<const uint8_t*>(&L.nodes[0]);
-        const uint8_t* meta = reinterpret_cast<const uint8_t*>(&L.nodes[1]) + 8;
-        const uint8_t* q = reinterpret_cast<const uint8_t*>(&L.nodes[2]);
-        const float p[3] = {L.nodes[0].x, L.nodes[0].y, L.nodes[0].z};
+    if (kind == CWBVH_KIND_CWBVH) {
+        const uint8_t* n0 = reinterpret_cast<const uint8_t*>(&root[0]);
+        const uint8_t* meta = reinterpret_cast<const uint8_t*>(&root[1]) + 8;
+        const uint8_t* q = reinterpret_cast<const uint8_t*>(&root[2]);
+        const float p[3] = {root[0].x, root[0].y, root[0].z};
         for (int s = 0; s < 8; ++s) {
             if (!meta[s]) continue;
             for (int k = 0; k < 3; ++k) {
@@ -244,8 +245,8 @@ static void layout_bounds(const cwbvh_layout& L, float lo[3], float hi[3]) {
                 b[k] = std::max(b[k], (float)(p[k] + q[24 + k * 8 + s] * sc));
             }
         }
-    } else if (L.kind == CWBVH_KIND_BVH2 && L.nodes.size() >= 4) {
-        const cwbvh_float4* n = L.nodes.data();
+    } else if (kind == CWBVH_KIND_BVH2) {
+        const cwbvh_float4* n = root;
         const float blo[2][3] = {{n[0].x, n[0].z, n[2].x}, {n[1].x, n[1].z, n[2].z}};
         const float bhi[2][3] = {{n[0].y, n[0].w, n[2].y}, {n[1].y, n[1].w, n[2].w}};
         for (int c = 0; c < 2; ++c) {
@@ -256,14 +257,13 @@ static void layout_bounds(const cwbvh_layout& L, float lo[3], float hi[3]) {
             }
         }
     }
+    bool known = true;
+    for (int k = 0; k < 3; ++k) known = known && a[k] <= b[k] && std::isfinite(a[k]) && std::isfinite(b[k]);
     for (int k = 0; k < 3; ++k) {
-        if (!(a[k] <= b[k])) {
-            a[k] = 0.f;
-            b[k] = 1.f;
-        }
-        lo[k] = a[k];
-        hi[k] = b[k];
+        lo[k] = known ? a[k] : 0.f;
+        hi[k] = known ? b[k] : 1.f;
     }
+    return known;
 }
 
 int32_t cwbvh_scene_create(const cwbvh_layout* L, void* stream, cwbvh_scene** out) {
@@ -306,7 +306,8 @@ int32_t cwbvh_scene_create_ex(const cwbvh_layout* L, void* stream, int32_t strid
     s->node_stride = stride;
     s->stack_need = L->stack_need;
     s->owning = true;
-    layout_bounds(*L, s->lo, s->hi);
+    if (L->nodes.size() >= (size_t)(L->kind == CWBVH_KIND_CWBVH ? 5 : 4))
+        s->bounds_known = root_bounds(L->kind, L->nodes.data(), s->lo, s->hi);
     s->woop_f4 = (int64_t)L->woop.size();
     hipGetDevice(&s->device);
     const size_t nb = node_f4 * sizeof(cwbvh_float4);
@@ -363,6 +364,12 @@ int32_t cwbvh_scene_wrap_device(int32_t kind, int32_t node_stride, const void* d
     s->stack_need = stack_need;
     s->owning = false;
     hipGetDevice(&s->device);
+    // bounds for the Morton ray pre-pass from the root node (a failed read
+    // leaves them unknown and the pre-pass off)
+    cwbvh_float4 root[5];
+    const int root_f4 = kind == CWBVH_KIND_CWBVH ? 5 : 4;
+    if (hipMemcpy(root, d_nodes, root_f4 * sizeof(cwbvh_float4), hipMemcpyDeviceToHost) == hipSuccess)
+        s->bounds_known = root_bounds(kind, root, s->lo, s->hi);
     // valid extent of the Woop array: up to the end of its allocation
     hipDeviceptr_t base = nullptr;
     size_t size = 0;
@@ -453,15 +460,15 @@ int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_confi
     // best with the speculative schedule (C3: 1.05x over grouped) and node
     // prefetch, whose loads of the next node overlap the triangle tests
     // (C3, 16.8 M rays: 1.07x; deeper LDS stacks and other refill budgets
-    // add nothing on top).  Wrapped
-    // device arrays carry no scene bounds to quantize origins in; unsorted
-    // rays keep the grouped schedule, which tests a leaf's triangles right
-    // after fetching the node (C3 shuffled: 1.10x over interleaved).
+    // add nothing on top).  A tree without bounds (an empty root) cannot
+    // quantize origins; its unsorted rays keep the grouped schedule, which
+    // tests a leaf's triangles right after fetching the node (C3 shuffled:
+    // 1.10x over interleaved).
     if (scene->bytes > kCacheResidentBytes) {
-        if (scene->owning) c->ray_order = 1;
+        if (scene->bounds_known) c->ray_order = 1;
         if (scene->kind == CWBVH_KIND_CWBVH) {
-            c->loop_mode = scene->owning ? 2 : 0;
-            c->prefetch = scene->owning ? 1 : 0;
+            c->loop_mode = scene->bounds_known ? 2 : 0;
+            c->prefetch = scene->bounds_known ? 1 : 0;
         }
     }
     return CWBVH_OK;
@@ -499,7 +506,13 @@ static int32_t trace_impl(const cwbvh_scene* scene, const cwbvh_ray* d_rays, con
         // the traversal that reads it is queued behind the sort
         auto lock = lock_stream_scratch();
         rc = morton_order_scratch(d_rays, n, scene->lo, scene->hi, L.stream, &L.order);
-        if (rc) return rc;
+        if (rc == CWBVH_ERR_OUT_OF_MEMORY) {
+            // no room for the sort's workspace: trace in the given order
+            L.order = nullptr;
+            set_error("");
+        } else if (rc) {
+            return rc;
+        }
         return scene->kind == CWBVH_KIND_CWBVH ? launch_cwbvh(L) : launch_bvh2(L);
     }
     return scene->kind == CWBVH_KIND_CWBVH ? launch_cwbvh(L) : launch_bvh2(L);

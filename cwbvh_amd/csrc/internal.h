@@ -22,6 +22,7 @@ struct cwbvh_scene {
     int32_t device = 0;
     float lo[3] = {0.f, 0.f, 0.f};  // scene bounds (Morton ordering); unit cube if unknown
     float hi[3] = {1.f, 1.f, 1.f};
+    bool bounds_known = false;      // lo/hi come from the root node's child boxes
 };
 
 namespace cw {

@@ -311,7 +311,8 @@ int32_t cwbvh_scene_create_ex(const cwbvh_layout* layout, void* stream, int32_t 
 /* float4 per node of the scene's device node array (BVH2 4, CWBVH 5 or 8). */
 int32_t cwbvh_scene_node_stride(const cwbvh_scene* scene, int32_t* stride);
 /* Wrap device arrays without taking ownership (bind semantics). prim_idx may
- * be NULL; stack_need = traversal stack entries the tree needs (0 = unknown). */
+ * be NULL; stack_need = traversal stack entries the tree needs (0 = unknown).
+ * The root node is read back once for the scene bounds. */
 int32_t cwbvh_scene_wrap_device(int32_t kind, int32_t node_stride, const void* d_nodes,
                                 const void* d_woop, const int32_t* d_prim_idx,
                                 int32_t stack_need, cwbvh_scene** out);
@@ -355,10 +356,12 @@ int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
 
 /* The configuration cwbvh_trace uses when cfg is NULL: the kind's default,
  * except for scenes larger than the 256 MB Infinity Cache, whose traversal is
- * bound by HBM traffic.  A scene uploaded from a host layout then gets the
- * Morton ray pre-pass (ray_order 1) and, for CWBVH, the speculative schedule
- * with node prefetch (loop_mode 2, prefetch 1); a wrapped one (no bounds)
- * gets the grouped schedule (loop_mode 0). */
+ * bound by HBM traffic.  Such a scene gets the Morton ray pre-pass
+ * (ray_order 1, in bounds read from its root node) and, for CWBVH, the
+ * speculative schedule with node prefetch (loop_mode 2, prefetch 1); one
+ * whose root holds no child box gets the grouped schedule (loop_mode 0).
+ * When the pre-pass cannot allocate its workspace, the launch traces the rays
+ * in the given order. */
 int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_config* cfg);
 
 /* Traverse n rays.  d_counters (nullable, CWBVH_TRACE_STATS only) receives
@@ -403,7 +406,8 @@ int32_t cwbvh_stack_overflowed(void* stream, int32_t* flag);
  * utilization of each part of the inner loop. */
 int32_t cwbvh_loop_profile(void* stream, uint64_t* out6);
 
-/* Scene bounds (lo[3], hi[3]) as stored at creation (wrapped scenes: unit cube). */
+/* Scene bounds (lo[3], hi[3]): the union of the root node's child boxes, read
+ * at creation (the unit cube when the root holds no child box). */
 int32_t cwbvh_scene_bounds(const cwbvh_scene* scene, float* lo3, float* hi3);
 
 /* Morton ray order: d_order[i] = index of the i-th ray when rays are sorted by

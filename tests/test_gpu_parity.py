@@ -267,6 +267,10 @@ def test_wrapped_scene_and_scene_default_config(cw, c0, c0_ref):
                            lay.stats()["stack_need"], keep=bufs)
         for any_hit in (False, True):
             assert_bitwise(sc.trace_numpy(wl.rays, int(any_hit)), c0_ref[(k, any_hit)], f"wrapped {k} any={any_hit}")
+        # bounds read back from the wrapped root equal the upload's
+        lo_w, hi_w = sc.bounds()
+        lo_o, hi_o = cw.Scene(lay).bounds()
+        assert np.array_equal(lo_w, lo_o) and np.array_equal(hi_w, hi_o), (lo_w, lo_o, hi_w, hi_o)
     small = cw.Scene(lays["cwbvh"]).default_config()
     assert small.loop_mode == cw.default_trace_config(cw.KIND_CWBVH).loop_mode == 1
     assert small.ray_order == 0
@@ -319,6 +323,19 @@ def test_large_scene_uploads_128b_nodes(cw):
         a = auto.trace_numpy(rays, int(any_hit))
         b = keep.trace_numpy(rays, int(any_hit), as_given)
         assert_bitwise(a, b, f"128 B nodes, sorted vs 80 B nodes, given order, any={any_hit}")
+    # the reference entry points on the same arrays (80 B nodes): the wrapped
+    # scene reads its bounds from the root, so rtTraceCWBVH sorts too
+    bufs = [cw.DeviceBuffer.from_numpy(a) for a in (cwb.nodes, cwb.woop, cwb.prim_idx)]
+    wrapped = cw.Scene.wrap(cw.KIND_CWBVH, 5, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, st["stack_need"], keep=bufs)
+    d = wrapped.default_config()
+    assert d.ray_order == 1 and d.loop_mode == 2 and d.prefetch == 1
+    n = len(rays)
+    d_rays = cw.DeviceBuffer.from_numpy(rays)
+    d_hits = cw.DeviceBuffer(16 * n)
+    say("large scene through rtBindCWBVHData / rtTraceCWBVH")
+    cw.rtBindCWBVHData(*(b.ptr for b in bufs))
+    cw.rtTraceCWBVH(d_rays.ptr, d_hits.ptr, n)
+    assert_bitwise(d_hits.download((n, 4)), keep.trace_numpy(rays, 0, as_given), "rtTraceCWBVH sorted vs given order")
     s2 = cw.Scene(bvh2)
     assert s2.default_config().ray_order == 1
     b2_given = s2.default_config()
