@@ -57,6 +57,7 @@ struct Tree {
     std::vector<BNode> nodes;
     std::vector<int32_t> refs;
     int32_t root = 0;
+    bool split_refs = false;  // built with spatial splits: a leaf can name a triangle twice
 };
 
 // A triangle reference with its box and centroid, so the recursive passes
@@ -607,23 +608,39 @@ struct Builder {
     }
 };
 
+bool finite_vertex(const V3& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
+
 void build_tree(const cwbvh_mesh* mesh, const cwbvh_build_params& P, Tree& T) {
-    int64_t nt = mesh->nt();
+    const int64_t mesh_nt = mesh->nt();
     int threads = num_threads_or_default(P.num_threads);
+    // A triangle with a NaN or infinite coordinate stays out of the tree:
+    // woopify gives it rows that never report a hit, and its box would make
+    // every ancestor's box (and the quantized child boxes of a CWBVH node)
+    // infinite, hiding the triangles beside it.
+    std::vector<int32_t> valid;
+    valid.reserve((size_t)mesh_nt);
+    for (int64_t t = 0; t < mesh_nt; ++t) {
+        V3 a, bb, c;
+        mesh->tri(t, a, bb, c);
+        if (finite_vertex(a) && finite_vertex(bb) && finite_vertex(c)) valid.push_back((int32_t)t);
+    }
+    const int64_t nt = (int64_t)valid.size();
     // room for the references spatial splits add
     const int64_t cap = P.split_alpha > 0.0f ? nt + (int64_t)((double)nt * P.split_budget) : nt;
+    T.split_refs = cap > nt;
     std::vector<PrimRef> refs((size_t)cap);
     parallel_for(nt, threads, [&](int64_t b, int64_t e) {
-        for (int64_t t = b; t < e; ++t) {
+        for (int64_t i = b; i < e; ++i) {
+            const int32_t t = valid[(size_t)i];
             V3 a, bb, c;
             mesh->tri(t, a, bb, c);
-            PrimRef& r = refs[(size_t)t];
+            PrimRef& r = refs[(size_t)i];
             r.box = Box();
             r.box.grow(a);
             r.box.grow(bb);
             r.box.grow(c);
             r.cen = r.box.center();
-            r.id = (int32_t)t;
+            r.id = t;
         }
     });
     T.nodes.assign(std::max<int64_t>(1, 2 * cap - 1), BNode());
@@ -840,7 +857,7 @@ void layout_bvh2(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_params
     const auto& N = T.nodes;
     const int32_t nn = (int32_t)N.size();
     Bvh2Layout B(mesh, T, L, num_threads_or_default(P.num_threads));
-    B.dup = T.refs.size() > (size_t)mesh->nt();
+    B.dup = T.split_refs;
     // bottom-up SAH: make a leaf wherever it is cheaper (children indices > parent)
     std::vector<double> cost(nn);
     B.leaf.assign(nn, 0);
@@ -1267,7 +1284,7 @@ void layout_cwbvh(const cwbvh_mesh* mesh, const Tree& T, const cwbvh_build_param
     CL.run(threads);
     CwbvhLayout W(mesh, T, CL, P, L, threads);
     W.root_area = T.nodes[T.root].box.area();
-    W.dup = T.refs.size() > (size_t)mesh->nt();
+    W.dup = T.split_refs;
     W.sub_nodes.assign(T.nodes.size(), 0);
     W.sub_tris.assign(T.nodes.size(), 0);
     W.count(T.root);

@@ -616,6 +616,30 @@ def test_spatial_split_layouts(cw, orc, split_scene, kind, kw):
     assert not cw.stack_overflowed()
 
 
+def test_non_finite_vertices(cw, orc):
+    """A mesh with NaN, +inf and -inf corners: the triangles they touch stay
+    out of the tree, and both kernels return the oracle's hits bit for bit,
+    closest and any hit."""
+    rng = np.random.default_rng(31)
+    v = rng.random((300, 3)).astype(np.float32)
+    idx = rng.integers(0, 300, (400, 3)).astype(np.int32)
+    v[5, 1], v[7, 0], v[9, 2] = np.nan, np.inf, -np.inf
+    bvh2, cwb = cw.build_layouts(cw.Mesh.from_arrays(v, idx))
+    rays = np.zeros((4096, 8), np.float32)
+    rays[:, :3] = rng.uniform(-0.2, 1.2, size=(4096, 3))
+    d = rng.normal(size=(4096, 3))
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 7] = np.inf
+    for lay in (cwb, bvh2):
+        sc = cw.Scene(lay)
+        for any_hit in (False, True):
+            say(f"non-finite vertices kind={lay.kind} any={any_hit}")
+            g = sc.trace_numpy(rays, int(any_hit))
+            ref = oracle_trace(orc, lay, rays, any_hit)
+            assert 0.1 < (orc.hit_ids(ref) >= 0).mean() < 0.9
+            assert_bitwise(g, ref, f"non-finite vertices kind={lay.kind} any={any_hit}")
+
+
 def test_stack_overflow_is_flagged_not_faulted(cw, orc):
     ray = np.array([[-1, -1, -1, 0, *(np.ones(3) / np.sqrt(3)), 1e30]], np.float32)
     # within capacity: traversal equals the oracle

@@ -394,6 +394,54 @@ def test_spatial_splits(cw, orc, stride):
     assert k_on[0] + k_on[1] < k_off[0] + k_off[1]
 
 
+def _mesh_with_non_finite_vertices(cw):
+    """400 random triangles over 300 vertices, three of them NaN, +inf and
+    -inf; returns the mesh and the mask of triangles whose corners are finite."""
+    rng = np.random.default_rng(31)
+    v = rng.random((300, 3)).astype(np.float32)
+    idx = rng.integers(0, 300, (400, 3)).astype(np.int32)
+    v[5, 1], v[7, 0], v[9, 2] = np.nan, np.inf, -np.inf
+    finite = ~np.isin(idx, [5, 7, 9]).any(axis=1)
+    return cw.Mesh.from_arrays(v, idx), v, idx, finite
+
+
+@pytest.mark.parametrize("split_alpha", [0.0, 1e-5])
+def test_non_finite_vertices_stay_out_of_the_tree(cw, orc, split_alpha):
+    """A triangle with a NaN or infinite corner can never be hit (its Woop
+    rows are the never-hit rows) and is left out of the tree, so its box does
+    not make the boxes around its neighbours infinite: both layouts hold
+    exactly the finite triangles, and traversal equals brute force over the
+    Woop rows of every triangle of the mesh, closest and any hit."""
+    mesh, v, idx, finite = _mesh_with_non_finite_vertices(cw)
+    assert 0 < (~finite).sum() < len(idx)
+    bvh2, cwb = cw.build_layouts(mesh, cw.default_build_params(split_alpha=split_alpha))
+    for lay in (bvh2, cwb):
+        present = set(int(p) for p in lay.prim_idx if p >= 0)
+        assert present == set(np.flatnonzero(finite).tolist())
+    rows = np.concatenate([cw.woopify(v[a], v[b], v[c]) for a, b, c in idx]).astype(np.float32)
+    assert np.array_equal(rows[3 * np.flatnonzero(~finite)], np.tile([[0, 0, 0, 1]], ((~finite).sum(), 1)))
+    rng = np.random.default_rng(32)
+    rays = np.zeros((3000, 8), np.float32)
+    rays[:, :3] = rng.uniform(-0.2, 1.2, size=(3000, 3))
+    d = rng.normal(size=(3000, 3))
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 7] = np.inf
+    addrs = np.arange(0, len(rows), 3, dtype=np.int32)
+    for any_hit in (False, True):
+        ref = orc.trace_brute(rows, addrs, rays, any_hit=any_hit)
+        hit_ref = orc.hit_ids(ref) >= 0
+        assert 0.1 < hit_ref.mean() < 0.9
+        for lay, got in ((cwb, orc.trace_cwbvh(cwb.nodes, cwb.node_stride, cwb.woop, rays, any_hit=any_hit)),
+                         (bvh2, orc.trace_bvh2(bvh2.nodes, bvh2.woop, rays, any_hit=any_hit))):
+            ids = orc.hit_ids(got)
+            assert np.array_equal(ids >= 0, hit_ref), (lay.kind, any_hit)
+            if not any_hit:
+                # the same triangle at the same distance (the layouts store
+                # the triangles at their own addresses)
+                assert np.array_equal(got[hit_ref, 0].view(np.uint32), ref[hit_ref, 0].view(np.uint32))
+                assert np.array_equal(lay.prim_idx[ids[hit_ref]], orc.hit_ids(ref)[hit_ref] // 3)
+
+
 def test_spatial_split_budget_and_params(cw):
     mesh = _slivers(cw, 300, 2000, seed=14)
     nt = mesh.num_triangles
