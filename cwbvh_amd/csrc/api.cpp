@@ -505,15 +505,23 @@ static int32_t trace_impl(const cwbvh_scene* scene, const cwbvh_ray* d_rays, con
         // the order lives in the stream's workspace: keep it locked until
         // the traversal that reads it is queued behind the sort
         auto lock = lock_stream_scratch();
-        rc = morton_order_scratch(d_rays, n, scene->lo, scene->hi, L.stream, &L.order);
-        if (rc == CWBVH_ERR_OUT_OF_MEMORY) {
-            // no room for the sort's workspace: trace in the given order
-            L.order = nullptr;
-            set_error("");
-        } else if (rc) {
-            return rc;
+        for (int64_t b = 0; b < n; b += kMortonMaxRays) {
+            TraceLaunch P = L;
+            P.rays = d_rays + b;
+            P.hits = d_hits + b;
+            P.n = std::min(n - b, kMortonMaxRays);
+            rc = morton_order_scratch(P.rays, P.n, scene->lo, scene->hi, L.stream, &P.order);
+            if (rc == CWBVH_ERR_OUT_OF_MEMORY) {
+                // no room for the sort's workspace: trace in the given order
+                P.order = nullptr;
+                set_error("");
+            } else if (rc) {
+                return rc;
+            }
+            rc = scene->kind == CWBVH_KIND_CWBVH ? launch_cwbvh(P) : launch_bvh2(P);
+            if (rc) return rc;
         }
-        return scene->kind == CWBVH_KIND_CWBVH ? launch_cwbvh(L) : launch_bvh2(L);
+        return CWBVH_OK;
     }
     return scene->kind == CWBVH_KIND_CWBVH ? launch_cwbvh(L) : launch_bvh2(L);
 }

@@ -68,6 +68,9 @@ int32_t morton_order_scratch(const void* d_rays, int64_t n, const float lo[3], c
 // cwbvh_trace_config.ray_order = 1 sorts launches of at least this many rays
 // (below it the sort's fixed cost outweighs the gain)
 constexpr int64_t kMortonMinRays = int64_t(1) << 20;
+// ... and sorts and traces larger launches in pieces of at most this many
+// rays, which bounds the kept workspace (about 1.1 GB)
+constexpr int64_t kMortonMaxRays = int64_t(1) << 26;
 
 struct TraceLaunch {
     const cwbvh_scene* scene;

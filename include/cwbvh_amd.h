@@ -347,9 +347,10 @@ typedef struct cwbvh_trace_config {
                                 7 = 7 waves per SIMD (72 VGPRs) */
     int32_t ray_order;       /* 0 = trace rays in the given order; 1 = a launch of 2^20 rays or more
                                 first sorts them by origin Morton code (as cwbvh_morton_order, in the
-                                stream's workspace) and traces them in that order.  Hits still go to
-                                each ray's own slot, so results do not change.  Applies to
-                                cwbvh_trace; ignored when an order is given and by surfel traces */
+                                stream's workspace) and traces them in that order, in pieces of at
+                                most 2^26 rays.  Hits still go to each ray's own slot, so results do
+                                not change.  Applies to cwbvh_trace; ignored when an order is given
+                                and by surfel traces */
 } cwbvh_trace_config;
 
 int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);

@@ -336,6 +336,17 @@ def test_large_scene_uploads_128b_nodes(cw):
     cw.rtBindCWBVHData(*(b.ptr for b in bufs))
     cw.rtTraceCWBVH(d_rays.ptr, d_hits.ptr, n)
     assert_bitwise(d_hits.download((n, 4)), keep.trace_numpy(rays, 0, as_given), "rtTraceCWBVH sorted vs given order")
+    # a launch above 2^26 rays is sorted and traced in pieces (here 2^26 and
+    # 4,321 rays); the rays repeat the set above, so do the hits
+    n_big = (1 << 26) + 4321
+    say(f"large scene, {n_big} rays in two sorted pieces")
+    reps = -(-n_big // len(rays))
+    big = auto.trace_numpy(np.tile(rays, (reps, 1))[:n_big], 0)
+    small = keep.trace_numpy(rays, 0, as_given)
+    for k in range(reps):
+        part = big[k * len(rays):(k + 1) * len(rays)]
+        assert_bitwise(part, small[:len(part)], f"piecewise sorted launch, repeat {k}")
+    del big
     s2 = cw.Scene(bvh2)
     assert s2.default_config().ray_order == 1
     b2_given = s2.default_config()
