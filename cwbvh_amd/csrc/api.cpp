@@ -23,6 +23,9 @@ const char* get_error() { return g_error.c_str(); }
 
 int32_t hip_fail(hipError_t e, const char* what) {
     set_error(std::string(what) + ": " + hipGetErrorString(e));
+    // the error is reported here: clear it from the runtime's last error so
+    // that the launch check of a later call does not report it again
+    (void)hipGetLastError();
     return e == hipErrorOutOfMemory ? CWBVH_ERR_OUT_OF_MEMORY
                                     : (e == hipErrorNoDevice ? CWBVH_ERR_NO_DEVICE : CWBVH_ERR_HIP);
 }
