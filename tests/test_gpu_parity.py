@@ -627,6 +627,18 @@ def test_spatial_split_layouts(cw, orc, split_scene, kind, kw):
     assert not cw.stack_overflowed()
 
 
+def test_failed_allocation_does_not_poison_later_calls(cw, c0, c0_ref):
+    """A device allocation that cannot succeed (1 PiB) fails with
+    CWBVH_ERR_OUT_OF_MEMORY; the error is not reported again by the next
+    trace, which returns the oracle's hits."""
+    wl, lays = c0
+    sc = cw.Scene(lays["cwbvh"])
+    with pytest.raises(cw.CwbvhError) as e:
+        cw.DeviceBuffer(1 << 50)
+    assert e.value.status == 3
+    assert_bitwise(sc.trace_numpy(wl.rays, 0), c0_ref[("cwbvh", False)], "trace after a failed allocation")
+
+
 def test_non_finite_vertices(cw, orc):
     """A mesh with NaN, +inf and -inf corners: the triangles they touch stay
     out of the tree, and both kernels return the oracle's hits bit for bit,
