@@ -51,6 +51,8 @@ int device_compute_units() {
         hipDeviceProp_t prop;
         if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
             cus = prop.multiProcessorCount;
+        else
+            (void)hipGetLastError();  // tolerated: fall back to one unit
         if (cus <= 0) cus = 1;
     });
     return cus;

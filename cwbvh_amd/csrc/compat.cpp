@@ -47,8 +47,11 @@ void trace(cwbvh_scene* scene, cwbvh_ray* rays, cwbvh_hit* hits, int count, cons
         return;
     }
     hipEvent_t a = nullptr, b = nullptr;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
-        g_status = CWBVH_ERR_HIP;
+    hipError_t ce = hipEventCreate(&a);
+    if (ce == hipSuccess) ce = hipEventCreate(&b);
+    if (ce != hipSuccess) {
+        if (a) hipEventDestroy(a);
+        g_status = cw::hip_fail(ce, "rtTrace event");
         return;
     }
     hipEventRecord(a, nullptr);
