@@ -479,12 +479,16 @@ int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_confi
     return CWBVH_OK;
 }
 
+// Rays and hits are read and written as 16-byte vectors.
+static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
 static int32_t trace_impl(const cwbvh_scene* scene, const cwbvh_ray* d_rays, const int32_t* d_order,
                           cwbvh_hit* d_hits, int64_t n, uint32_t flags, const cwbvh_trace_config* cfg,
                           uint64_t* d_counters, void* stream) {
     CW_CHECK_ARG(scene != nullptr, "scene is NULL");
     CW_CHECK_ARG(n >= 0 && n <= 0x7FFFFFFF, "ray count must be in [0, 2^31)");
     CW_CHECK_ARG(n == 0 || (d_rays != nullptr && d_hits != nullptr), "NULL ray or hit buffer");
+    CW_CHECK_ARG(aligned16(d_rays) && aligned16(d_hits), "ray and hit buffers must be 16-byte aligned");
     CW_CHECK_ARG((flags & ~(uint32_t)(CWBVH_TRACE_ANY | CWBVH_TRACE_PRIM_ID | CWBVH_TRACE_STATS)) == 0, "unknown flags");
     CW_CHECK_ARG(!(flags & CWBVH_TRACE_STATS) || d_counters != nullptr, "STATS needs a counter buffer");
     CW_CHECK_ARG(!(flags & CWBVH_TRACE_PRIM_ID) || scene->prim_idx != nullptr, "PRIM_ID needs a prim index array");
@@ -545,6 +549,8 @@ int32_t cwbvh_rays_ao_from_samples_device(const cwbvh_float4* d_positions, const
     CW_CHECK_ARG(num_texels >= 0 && rays_per_texel >= 1, "bad ray count");
     CW_CHECK_ARG(num_texels * (int64_t)rays_per_texel <= 0x7FFFFFFF, "more than 2^31 - 1 rays");
     CW_CHECK_ARG(num_texels == 0 || (d_positions && d_normals && d_radius && d_out), "NULL argument");
+    CW_CHECK_ARG(aligned16(d_positions) && aligned16(d_normals) && aligned16(d_out),
+                 "surfel and ray arrays must be 16-byte aligned");
     return launch_surfel_rays(d_positions, d_normals, d_radius, num_texels, rays_per_texel, tmin, tmax, seed, d_out,
                               (hipStream_t)stream);
 }
@@ -559,6 +565,8 @@ int32_t cwbvh_trace_surfels(const cwbvh_scene* scene, const cwbvh_float4* d_posi
     const int64_t n = num_texels * (int64_t)rays_per_texel;
     CW_CHECK_ARG(n <= 0x7FFFFFFF, "more than 2^31 - 1 rays");
     CW_CHECK_ARG(n == 0 || (d_positions && d_normals && d_radius), "NULL surfel array");
+    CW_CHECK_ARG(aligned16(d_positions) && aligned16(d_normals) && aligned16(d_hits),
+                 "surfel and hit arrays must be 16-byte aligned");
     CW_CHECK_ARG((d_hits != nullptr) != (d_texel_hits != nullptr), "give exactly one of hits and per-surfel counts");
     CW_CHECK_ARG((flags & ~(uint32_t)(CWBVH_TRACE_ANY | CWBVH_TRACE_PRIM_ID | CWBVH_TRACE_STATS)) == 0, "unknown flags");
     CW_CHECK_ARG(!(flags & CWBVH_TRACE_PRIM_ID) || (d_hits && scene->prim_idx), "PRIM_ID needs hits and a prim index");

@@ -278,9 +278,11 @@ int32_t cwbvh_batch_create(const cwbvh_batch_item* items, int32_t count, uint32_
         const cwbvh_batch_item& it = items[i];
         if (!it.scene || it.scene->kind != CWBVH_KIND_CWBVH || it.n < 0 || it.n > 0x7FFFFFFF ||
             (it.n > 0 && (!it.d_rays || !it.d_hits)) ||
+            (((uintptr_t)it.d_rays | (uintptr_t)it.d_hits) & 15u) != 0 ||
             ((flags & CWBVH_TRACE_PRIM_ID) && !it.scene->prim_idx)) {
             delete b;
-            set_error("batch item " + std::to_string(i) + " is invalid (needs a CWBVH scene, rays and hits)");
+            set_error("batch item " + std::to_string(i) +
+                      " is invalid (needs a CWBVH scene and 16-byte aligned rays and hits)");
             return CWBVH_ERR_INVALID_ARGUMENT;
         }
         if (it.n == 0) continue;

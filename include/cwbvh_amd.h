@@ -366,7 +366,8 @@ int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
 int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_config* cfg);
 
 /* Traverse n rays.  d_counters (nullable, CWBVH_TRACE_STATS only) receives
- * uint64[2] = {visited nodes, tested triangles}, accumulated. */
+ * uint64[2] = {visited nodes, tested triangles}, accumulated.  Ray and hit
+ * buffers must be 16-byte aligned (CWBVH_ERR_INVALID_ARGUMENT otherwise). */
 int32_t cwbvh_trace(const cwbvh_scene* scene, const cwbvh_ray* d_rays, cwbvh_hit* d_hits,
                     int64_t n, uint32_t flags, const cwbvh_trace_config* cfg,
                     uint64_t* d_counters, void* stream);

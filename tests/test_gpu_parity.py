@@ -372,6 +372,21 @@ def test_invalid_loop_configs_rejected(cw, c0):
         assert e.value.status == 1, kw
 
 
+def test_misaligned_buffers_rejected(cw, c0):
+    """Rays and hits are read and written as 16-byte vectors: buffers that
+    are not 16-byte aligned are refused before any launch."""
+    wl, lays = c0
+    sc = cw.Scene(lays["cwbvh"])
+    dr = cw.DeviceBuffer.from_numpy(wl.rays[:65])
+    dh = cw.DeviceBuffer(16 * 65)
+    for rays, hits in ((dr.ptr + 4, dh.ptr), (dr.ptr, dh.ptr + 8)):
+        with pytest.raises(cw.CwbvhError) as e:
+            sc.trace(rays, hits, 64, 0)
+        assert e.value.status == 1
+    with pytest.raises(cw.CwbvhError):
+        cw.Batch([(sc, dr.ptr + 16 + 4, dh.ptr, 64)])
+
+
 def test_prim_id_output(cw, c0, c0_ref):
     wl, lays = c0
     for k, lay in lays.items():
