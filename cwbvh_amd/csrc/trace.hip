@@ -313,6 +313,7 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
         if (got) {
             const Ray4* rays = P.rays;
             const int* order = P.order;
+            unsigned ray_count = (unsigned)P.n;
             if (MULTI) {
                 const cw::BatchJob& J = P.jobs[my_job];
                 nodes = (const float4*)J.nodes;
@@ -323,6 +324,7 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                 spec_limit = J.spec_limit;
                 rays = (const Ray4*)J.rays;
                 order = J.order;
+                ray_count = (unsigned)J.n;
             }
             if constexpr (SRC == kRaysFromSurfels) {
                 // the ray is made here from its surfel instead of read back
@@ -338,7 +340,10 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                 tmin = P.gen_tmin;
                 tmax = P.gen_tmax;
             } else {
-                const int ri = order ? order[my] : (int)my;
+                int ri = order ? order[my] : (int)my;
+                // an order entry outside the launch's rays traces ray `my`
+                // itself, so a malformed order cannot read or write out of bounds
+                if ((unsigned)ri >= ray_count) ri = (int)my;
                 out_idx = ri;
                 // rays are read once: keep them out of the caches the BVH lives in
                 const float4 ro = nt_load4(&rays[ri].o);
@@ -620,7 +625,8 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace(Bvh2Params P) {
     int spill[kBvh2StackCapacity];
     const StackIndex sx((int)threadIdx.x, P.lds_depth);
 
-    const int ri = P.order ? P.order[i] : i;
+    int ri = P.order ? P.order[i] : i;
+    if ((unsigned)ri >= (unsigned)P.n) ri = i;  // malformed order: stay in bounds
     const float4 ro = nt_load4(&P.rays[ri].o);
     const float4 rd = nt_load4(&P.rays[ri].d);
     const float3 o = make_float3(ro.x, ro.y, ro.z), d = make_float3(rd.x, rd.y, rd.z);

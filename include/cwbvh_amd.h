@@ -372,7 +372,8 @@ int32_t cwbvh_trace(const cwbvh_scene* scene, const cwbvh_ray* d_rays, cwbvh_hit
                     int64_t n, uint32_t flags, const cwbvh_trace_config* cfg,
                     uint64_t* d_counters, void* stream);
 /* Same, with a ray permutation: ray i of the launch is d_rays[d_order[i]] and
- * its hit goes to d_hits[d_order[i]]. */
+ * its hit goes to d_hits[d_order[i]].  An entry outside [0, n) traces ray i
+ * into d_hits[i] instead, so no access leaves the buffers. */
 int32_t cwbvh_trace_ordered(const cwbvh_scene* scene, const cwbvh_ray* d_rays,
                             const int32_t* d_order, cwbvh_hit* d_hits, int64_t n,
                             uint32_t flags, const cwbvh_trace_config* cfg,

@@ -387,6 +387,25 @@ def test_misaligned_buffers_rejected(cw, c0):
         cw.Batch([(sc, dr.ptr + 16 + 4, dh.ptr, 64)])
 
 
+@pytest.mark.parametrize("kw", [None, dict(persistent=0), dict(loop_mode=2, prefetch=1)])
+def test_order_entries_out_of_range_stay_in_bounds(cw, c0, c0_ref, kw):
+    """An ordered trace whose order holds entries outside [0, n) traces ray i
+    into slot i for those entries: with an otherwise identity order the hits
+    equal the plain trace's, and no access leaves the buffers."""
+    wl, lays = c0
+    n = len(wl.rays)
+    order = np.arange(n, dtype=np.int32)
+    order[[3, 100, 777, n - 1]] = [-5, n, 1 << 30, -(1 << 31)]
+    for k in ("cwbvh", "bvh2"):
+        if kw is not None and k == "bvh2":
+            continue
+        say(f"out-of-range order entries {k} {kw}")
+        sc = cw.Scene(lays[k])
+        cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw) if kw is not None else None
+        g = sc.trace_numpy(wl.rays, 0, cfg, order=order)
+        assert_bitwise(g, c0_ref[(k, False)], f"out-of-range order entries {k} {kw}")
+
+
 def test_prim_id_output(cw, c0, c0_ref):
     wl, lays = c0
     for k, lay in lays.items():
