@@ -270,6 +270,11 @@ def bench_c3(args, cw, W, dev) -> int:
             res[f"{name}_{oname}"] = {"ms": round(ms, 3), "mrays_per_s": round(n / ms / 1e3, 1)}
             log(f"{name} {oname}: {ms:.2f} ms {n / ms / 1e3:.0f} Mrays/s")
     log(f"morton sort {sort_ms:.2f} ms")
+    # the reference's 80 B nodes with their scene default, as rtTraceCWBVH
+    # traces caller arrays bound in that layout
+    ms80 = t(s_cw80, s_cw80.default_config(), None)
+    res["cwbvh_node80_default"] = {"ms": round(ms80, 3), "mrays_per_s": round(n / ms80 / 1e3, 1)}
+    log(f"cwbvh 80 B nodes, scene default: {ms80:.2f} ms {n / ms80 / 1e3:.0f} Mrays/s")
     # node layout x inner-loop schedule: this scene is far larger than L2 and
     # the Infinity Cache, so line-aligned nodes and access order matter
     layouts = []
