@@ -573,7 +573,12 @@ int32_t cwbvh_trace_surfels(const cwbvh_scene* scene, const cwbvh_float4* d_posi
     CW_CHECK_ARG(!(flags & CWBVH_TRACE_STATS) || d_counters != nullptr, "STATS needs a counter buffer");
     cwbvh_trace_config c;
     if (cfg) c = *cfg;
-    else cwbvh_trace_config_for_scene(scene, &c);
+    else {
+        // the scene's default without node prefetch, which surfel traces do
+        // not have (rays are made at refill; ray_order does not apply)
+        cwbvh_trace_config_for_scene(scene, &c);
+        c.prefetch = 0;
+    }
     int32_t rc = validate_config(scene->kind, c);
     if (rc) return rc;
     if (n == 0) return CWBVH_OK;

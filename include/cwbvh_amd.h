@@ -390,7 +390,8 @@ int32_t cwbvh_rays_ao_from_samples_device(const cwbvh_float4* d_positions, const
  * cwbvh_trace writes them for the generated rays) or, with d_hits NULL, the
  * number of rays of each surfel that hit something added to d_texel_hits[i]
  * (uint32 per surfel; the caller clears it).  Flags as cwbvh_trace (PRIM_ID
- * only with d_hits); node prefetch is not used. */
+ * only with d_hits).  Node prefetch is not used: cfg must have prefetch 0,
+ * and with cfg NULL the scene's default is taken without it. */
 int32_t cwbvh_trace_surfels(const cwbvh_scene* scene, const cwbvh_float4* d_positions,
                             const cwbvh_float4* d_normals, const float* d_radius, int64_t num_texels,
                             int32_t rays_per_texel, float tmin, float tmax, uint64_t seed, uint32_t flags,

@@ -347,6 +347,23 @@ def test_large_scene_uploads_128b_nodes(cw):
         part = big[k * len(rays):(k + 1) * len(rays)]
         assert_bitwise(part, small[:len(part)], f"piecewise sorted launch, repeat {k}")
     del big
+    # fused surfel trace with the scene's default (which has node prefetch,
+    # not used by surfel traces): equal to the same schedule without prefetch
+    rs = np.random.default_rng(43)
+    pos = np.zeros((2048, 4), np.float32)
+    pos[:, :3] = mesh.bounds()[0] + rs.random((2048, 3)) * (mesh.bounds()[1] - mesh.bounds()[0])
+    nrm = np.zeros((2048, 4), np.float32)
+    nv = rs.normal(size=(2048, 3))
+    nrm[:, :3] = nv / np.linalg.norm(nv, axis=1, keepdims=True)
+    bufs_s = [cw.DeviceBuffer.from_numpy(a) for a in (pos, nrm, np.full(2048, 1e-3, np.float32))]
+    hs = [cw.DeviceBuffer(16 * 2048 * 8) for _ in range(2)]
+    say("large scene fused surfel trace, default configuration")
+    auto.trace_surfels(*(b.ptr for b in bufs_s), 2048, 8, 0.0, np.inf, 7, d_hits=hs[0].ptr)
+    auto.trace_surfels(*(b.ptr for b in bufs_s), 2048, 8, 0.0, np.inf, 7, d_hits=hs[1].ptr,
+                       cfg=auto.default_config(prefetch=0))
+    a, b = (h.download((2048 * 8, 4)) for h in hs)
+    assert_bitwise(a, b, "surfel trace, scene default vs explicit")
+    assert (cw.hit_ids(a) >= 0).mean() > 0.05
     s2 = cw.Scene(bvh2)
     assert s2.default_config().ray_order == 1
     b2_given = s2.default_config()
