@@ -149,8 +149,9 @@ constexpr int kLoopSpeculate = 2;
 // millions of AO rays 3-5% faster and launches of a million rays 6-7% slower
 // (more waves, each slower, lengthen the tail of a short launch).  The other
 // instances stay unconstrained: the batch, counting and prefetching kernels
-// spill 10 to 32 registers at 7 waves, and the grouped schedule serves scenes
-// that stream from HBM, where spills and the extra wave cost more.
+// spill 10 to 32 registers at 7 waves, and the grouped and speculative
+// schedules serve scenes that stream from HBM, where spills and the extra
+// wave cost more.
 constexpr int kOccupancyAuto = 0;
 constexpr int kOccupancyFree = 1;
 constexpr int kOccupancy7 = 7;
