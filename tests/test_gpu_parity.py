@@ -423,6 +423,37 @@ def test_order_entries_out_of_range_stay_in_bounds(cw, c0, c0_ref, kw):
         assert_bitwise(g, c0_ref[(k, False)], f"out-of-range order entries {k} {kw}")
 
 
+def test_sorted_launches_on_two_threads_and_streams(cw, c0, c0_ref):
+    """ray_order = 1 on any scene sorts launches of 2^20 rays or more in the
+    stream's own workspace: two host threads tracing on two streams at once
+    (different sizes, several rounds) each get the hits of the given order."""
+    import concurrent.futures as cf
+    wl, lays = c0
+    sc = cw.Scene(lays["cwbvh"])
+    base = len(wl.rays)
+    ref = c0_ref[("cwbvh", False)]
+    cfg = sc.default_config(ray_order=1)
+    sizes = ((1 << 20) + 77, (1 << 20) + 3 * base // 2)
+    jobs = []
+    for n in sizes:
+        reps = -(-n // base)
+        rays = np.tile(wl.rays, (reps, 1))[:n]
+        jobs.append((n, cw.DeviceBuffer.from_numpy(rays), cw.DeviceBuffer(16 * n), cw.Stream(),
+                     np.tile(ref, (reps, 1))[:n]))
+
+    def run(job):
+        n, dr, dh, st, want = job
+        for _ in range(3):
+            sc.trace(dr.ptr, dh.ptr, n, 0, cfg, None, st)
+        st.synchronize()
+        return dh.download((n, 4)), want
+
+    say("sorted launches on two threads")
+    with cf.ThreadPoolExecutor(2) as ex:
+        for got, want in ex.map(run, jobs):
+            assert_bitwise(got, want, "sorted launch on its own thread and stream")
+
+
 def test_prim_id_output(cw, c0, c0_ref):
     wl, lays = c0
     for k, lay in lays.items():
