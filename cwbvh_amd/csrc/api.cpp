@@ -522,10 +522,8 @@ static int32_t trace_impl(const cwbvh_scene* scene, const cwbvh_ray* d_rays, con
             rc = morton_order_scratch(P.rays, P.n, scene->lo, scene->hi, L.stream, &P.order);
             if (rc == CWBVH_ERR_OUT_OF_MEMORY) {
                 // no room for the sort's workspace: trace in the given order
-                // (and clear the failed allocation from the runtime's last
-                // error, which the launch checks)
+                // (hip_fail has already cleared the runtime's last error)
                 P.order = nullptr;
-                (void)hipGetLastError();
                 set_error("");
             } else if (rc) {
                 return rc;
