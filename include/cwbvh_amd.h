@@ -349,8 +349,9 @@ typedef struct cwbvh_trace_config {
                                 first sorts them by origin Morton code (as cwbvh_morton_order, in the
                                 stream's workspace) and traces them in that order, in pieces of at
                                 most 2^26 rays.  Hits still go to each ray's own slot, so results do
-                                not change.  Applies to cwbvh_trace; ignored when an order is given
-                                and by surfel traces */
+                                not change.  Applies to cwbvh_trace; ignored when an order is given,
+                                by surfel traces, and by batches and the server (batches have their
+                                own Morton pre-pass, the `morton` argument) */
 } cwbvh_trace_config;
 
 int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
