@@ -454,6 +454,27 @@ def test_sorted_launches_on_two_threads_and_streams(cw, c0, c0_ref):
             assert_bitwise(got, want, "sorted launch on its own thread and stream")
 
 
+def test_sorted_launch_prim_ids_and_counts(cw, c0):
+    """With the Morton pre-pass, PRIM_ID output equals the unsorted launch's
+    and the STATS totals (sums over rays of visited nodes and tested
+    triangles) are the same numbers: sorting only reorders the rays."""
+    wl, lays = c0
+    sc = cw.Scene(lays["cwbvh"])
+    n = (1 << 20) + 5
+    rays = np.tile(wl.rays, (-(-n // len(wl.rays)), 1))[:n]
+    dr = cw.DeviceBuffer.from_numpy(rays)
+    out = {}
+    for order in (0, 1):
+        say(f"PRIM_ID | STATS with ray_order={order}")
+        dh, dc = cw.DeviceBuffer(16 * n), cw.DeviceBuffer(16)
+        dc.zero()
+        sc.trace(dr.ptr, dh.ptr, n, cw.TRACE_PRIM_ID | cw.TRACE_STATS, sc.default_config(ray_order=order), dc.ptr)
+        cw.synchronize()
+        out[order] = (dh.download((n, 4)), dc.download((2,), np.uint64))
+    assert_bitwise(out[1][0], out[0][0], "PRIM_ID, sorted vs given order")
+    assert np.array_equal(out[1][1], out[0][1]) and out[0][1][0] > n
+
+
 def test_prim_id_output(cw, c0, c0_ref):
     wl, lays = c0
     for k, lay in lays.items():
