@@ -244,6 +244,7 @@ _SIGS = {
     "cwbvh_trace_config_for_scene": (_I32, [_P, _P]),
     "cwbvh_scene_bounds": (_I32, [_P, _P, _P]),
     "cwbvh_morton_order": (_I32, [_P, _I64, _P, _P, _P, _P]),
+    "cwbvh_set_workspace_limit": (_I32, [_I64]),
     "cwbvh_batch_create": (_I32, [_P, _I32, _U32, _P, _I32, _PP]),
     "cwbvh_batch_launch": (_I32, [_P, _P]),
     "cwbvh_batch_capture": (_I32, [_P, _P]),
@@ -841,6 +842,11 @@ def morton_order(d_rays: int, n: int, lo, hi, d_order: int, stream: Stream | Non
     lo = _f32(lo)
     hi = _f32(hi)
     check(lib.cwbvh_morton_order(d_rays, n, _ptr(lo), _ptr(hi), d_order, stream.handle if stream else None))
+
+
+def set_workspace_limit(nbytes: int) -> None:
+    """Largest sort workspace one stream may hold (0 = no limit)."""
+    check(lib.cwbvh_set_workspace_limit(int(nbytes)))
 
 
 def _items(items) -> C.Array:

@@ -73,6 +73,22 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+DROPIN_SRC = PKG.parent / "tests" / "dropin" / "main.cpp"
+DROPIN_BIN = PKG.parent / "tests" / "dropin" / "dropin_main"
+
+
+def build_dropin(out: Path = DROPIN_BIN) -> Path:
+    """Compile and link the C++ caller of the reference entry points
+    (tests/dropin/main.cpp) against include/ and libcwbvh_amd.so, as a
+    reference-side driver would be built."""
+    out = Path(out)
+    tmp = out.with_name(out.name + ".tmp")
+    _run([str(ROCM / "bin" / "hipcc"), "-O2", "-std=c++17", f"-I{PKG.parent / 'include'}", str(DROPIN_SRC),
+          "-o", str(tmp), f"-L{PKG}", "-lcwbvh_amd", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../../cwbvh_amd"])
+    os.replace(tmp, out)
+    return out
+
+
 def clean() -> None:
     shutil.rmtree(BUILD, ignore_errors=True)
     if LIB.exists():

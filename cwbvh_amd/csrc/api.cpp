@@ -59,10 +59,16 @@ int device_compute_units() {
 }
 
 // Keyed by (current device, stream): the null stream names a different queue
-// on each device, and each device needs counters in its own memory.
+// on each device, and each device needs counters in its own memory.  The
+// per-thread handle names a different stream on every host thread, so one key
+// cannot stand for it: it is refused.
 int32_t stream_counters(hipStream_t stream, unsigned** out) {
     static std::mutex mu;
     static std::map<std::pair<int, hipStream_t>, unsigned*> table;
+    if (stream == hipStreamPerThread) {
+        set_error("hipStreamPerThread is not supported: pass a stream handle or NULL");
+        return CWBVH_ERR_INVALID_ARGUMENT;
+    }
     int dev = 0;
     CW_HIP(hipGetDevice(&dev));
     const std::pair<int, hipStream_t> key(dev, stream);

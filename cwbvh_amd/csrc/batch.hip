@@ -128,8 +128,19 @@ size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 // never freed between a call's lookup and its launches.
 std::mutex g_scratch_mu;
 std::map<std::pair<int, hipStream_t>, std::pair<char*, size_t>> g_scratch;
+// cwbvh_set_workspace_limit: largest workspace one stream may hold (0 = no limit)
+size_t g_scratch_limit = 0;
 
 int32_t stream_scratch(hipStream_t st, size_t bytes, char** out) {
+    if (st == hipStreamPerThread) {
+        cw::set_error("hipStreamPerThread is not supported: pass a stream handle or NULL");
+        return CWBVH_ERR_INVALID_ARGUMENT;
+    }
+    if (g_scratch_limit > 0 && bytes > g_scratch_limit) {
+        cw::set_error("sort workspace of " + std::to_string(bytes) + " bytes exceeds the limit of " +
+                  std::to_string(g_scratch_limit));
+        return CWBVH_ERR_OUT_OF_MEMORY;
+    }
     int dev = 0;
     CW_HIP(hipGetDevice(&dev));
     auto& slot = g_scratch[std::make_pair(dev, st)];
@@ -139,7 +150,8 @@ int32_t stream_scratch(hipStream_t st, size_t bytes, char** out) {
             slot = {nullptr, 0};
         }
         // 1/8 headroom so slowly growing calls do not reallocate every time
-        const size_t want = bytes + bytes / 8;
+        size_t want = bytes + bytes / 8;
+        if (g_scratch_limit > 0) want = std::min(want, g_scratch_limit);
         CW_HIP(hipMalloc(&slot.first, want));
         slot.second = want;
     }
@@ -241,6 +253,23 @@ int32_t cwbvh_scene_bounds(const cwbvh_scene* s, float* lo3, float* hi3) {
     for (int k = 0; k < 3; ++k) {
         lo3[k] = s->lo[k];
         hi3[k] = s->hi[k];
+    }
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_set_workspace_limit(int64_t bytes) {
+    CW_CHECK_ARG(bytes >= 0, "limit must be >= 0");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    g_scratch_limit = (size_t)bytes;
+    // workspaces already above the new limit are released (hipFree waits for
+    // the work queued on them)
+    for (auto it = g_scratch.begin(); it != g_scratch.end();) {
+        if (bytes > 0 && it->second.second > (size_t)bytes) {
+            if (it->second.first) (void)hipFree(it->second.first);
+            it = g_scratch.erase(it);
+        } else {
+            ++it;
+        }
     }
     return CWBVH_OK;
 }

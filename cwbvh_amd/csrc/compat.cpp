@@ -8,7 +8,16 @@
 // stream, waits for completion, writes one hit per ray and logs
 // "<ms>MS, <rate>MRays/s (...)".  Instead of assert(0) on a failure, the
 // status is kept for cwbvh_last_status().
+//
+// Each entry point is exported twice:
+//  * with C++ linkage and the reference's own parameter types (HIP float4,
+//    global-namespace Ray and Hit), so the symbols carry the mangled names a
+//    caller compiled against src/TraversalKernelCWBVH.h / TraversalKernelBVH2.h
+//    links to (include/TraversalKernel*.h declare exactly these);
+//  * with C linkage over cwbvh_float4 / cwbvh_ray / cwbvh_hit (cwbvh_amd.h),
+//    for FFI callers that cannot bind mangled C++ names.
 #include <hip/hip_runtime_api.h>
+#include <hip/hip_vector_types.h>
 
 #include <cstdio>
 #include <mutex>
@@ -104,3 +113,38 @@ float cwbvh_last_trace_ms(void) { return g_ms; }
 void cwbvh_set_log(int32_t enabled) { g_log = enabled ? 1 : 0; }
 
 }  // extern "C"
+
+// The reference's ray and hit records (src/helper_math.h:1853-1865), named as
+// there so that the C++ entry points below mangle to the reference's symbols.
+struct Ray {
+    float4 origin_tmin;
+    float4 dir_tmax;
+};
+struct Hit {
+    float4 t_triId_u_v;
+};
+static_assert(sizeof(float4) == sizeof(cwbvh_float4) && sizeof(Ray) == sizeof(cwbvh_ray) &&
+                  sizeof(Hit) == sizeof(cwbvh_hit),
+              "the reference records and the C ABI records have one layout");
+
+void rtBindCWBVHData(const float4* InBVHTreeNodes, const float4* InTriangleWoopCoordinates,
+                     const int* InMappingFromTriangleAddressToIndex) {
+    rtBindCWBVHData(reinterpret_cast<const cwbvh_float4*>(InBVHTreeNodes),
+                    reinterpret_cast<const cwbvh_float4*>(InTriangleWoopCoordinates),
+                    InMappingFromTriangleAddressToIndex);
+}
+
+void rtTraceCWBVH(Ray* rayBuffer, Hit* rayResultBuffer, int rayCount) {
+    rtTraceCWBVH(reinterpret_cast<cwbvh_ray*>(rayBuffer), reinterpret_cast<cwbvh_hit*>(rayResultBuffer), rayCount);
+}
+
+void rtBindBVH2Data(const float4* InBVHTreeNodes, const float4* InTriangleWoopCoordinates,
+                    const int* InMappingFromTriangleAddressToIndex) {
+    rtBindBVH2Data(reinterpret_cast<const cwbvh_float4*>(InBVHTreeNodes),
+                   reinterpret_cast<const cwbvh_float4*>(InTriangleWoopCoordinates),
+                   InMappingFromTriangleAddressToIndex);
+}
+
+void rtTraceBVH2(Ray* rayBuffer, Hit* rayResultBuffer, int rayCount) {
+    rtTraceBVH2(reinterpret_cast<cwbvh_ray*>(rayBuffer), reinterpret_cast<cwbvh_hit*>(rayResultBuffer), rayCount);
+}

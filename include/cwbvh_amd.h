@@ -22,7 +22,10 @@
  *
  * Every function returns a cwbvh_status (0 = OK) unless stated otherwise.
  * Device pointers are plain HIP device addresses; "stream" arguments are
- * hipStream_t passed as void* (NULL = the null stream).
+ * hipStream_t passed as void* (NULL = the null stream).  Work counters and
+ * sort workspaces are kept per stream handle, so the special handle
+ * hipStreamPerThread (a different stream on every host thread) is refused
+ * with CWBVH_ERR_INVALID_ARGUMENT wherever a stream is taken.
  */
 #ifndef CWBVH_AMD_H
 #define CWBVH_AMD_H
@@ -324,7 +327,7 @@ void cwbvh_scene_destroy(cwbvh_scene* scene);
 
 typedef struct cwbvh_trace_config {
     int32_t block_size;      /* 64, 128 or 256 threads */
-    int32_t lds_depth;       /* stack entries per lane kept in LDS (0..stack capacity) */
+    int32_t lds_depth;       /* stack entries per lane kept in LDS (1..stack capacity) */
     int32_t grid_blocks;     /* persistent CWBVH grid (0 = fill the device) */
     int32_t fetch_chunk;     /* rays claimed per wave-level atomic (CWBVH; 0 = automatic) */
     int32_t refill_idle;     /* leave the inner loop once idle lanes exceed this ... */
@@ -423,6 +426,12 @@ int32_t cwbvh_scene_bounds(const cwbvh_scene* scene, float* lo3, float* hi3);
  * cwbvh_stream_destroy releases a stream's workspace. */
 int32_t cwbvh_morton_order(const cwbvh_ray* d_rays, int64_t n, const float* lo3, const float* hi3,
                            int32_t* d_order, void* stream);
+/* Largest sort workspace one stream may hold, in bytes (0 = no limit, the
+ * default).  Workspaces above a new limit are released.  A call that would
+ * need more fails with CWBVH_ERR_OUT_OF_MEMORY; a cwbvh_trace with ray_order 1
+ * then traces its rays in the given order instead, as when the device has no
+ * room for the workspace. */
+int32_t cwbvh_set_workspace_limit(int64_t bytes);
 
 /* ------------------------------------------- multi-scene batches --- */
 

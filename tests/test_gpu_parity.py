@@ -742,6 +742,52 @@ def test_failed_allocation_does_not_poison_later_calls(cw, c0, c0_ref):
     assert_bitwise(sc.trace_numpy(wl.rays, 0), c0_ref[("cwbvh", False)], "trace after a failed allocation")
 
 
+def test_sort_workspace_limit_falls_back_to_given_order(cw, c0, c0_ref):
+    """A ray_order = 1 launch whose sort workspace exceeds the workspace limit
+    traces its rays in the given order (same hits), leaves no error for the
+    next call, and a direct cwbvh_morton_order call fails with
+    CWBVH_ERR_OUT_OF_MEMORY under the limit."""
+    wl, lays = c0
+    sc = cw.Scene(lays["cwbvh"])
+    n = (1 << 20) + 11
+    reps = -(-n // len(wl.rays))
+    rays = np.tile(wl.rays, (reps, 1))[:n]
+    want = np.tile(c0_ref[("cwbvh", False)], (reps, 1))[:n]
+    dr, dh, do = cw.DeviceBuffer.from_numpy(rays), cw.DeviceBuffer(16 * n), cw.DeviceBuffer(4 * n)
+    st = cw.Stream()
+    cfg = sc.default_config(ray_order=1)
+    try:
+        cw.set_workspace_limit(1 << 20)   # far below the ~16 B per ray the sort needs
+        say("sorted launch under a 1 MiB workspace limit")
+        sc.trace(dr.ptr, dh.ptr, n, 0, cfg, None, st)
+        st.synchronize()
+        assert cw.lib.cwbvh_last_error().decode() == ""
+        assert_bitwise(dh.download((n, 4)), want, "sorted launch without room for the sort")
+        lo, hi = sc.bounds()
+        with pytest.raises(cw.CwbvhError) as e:
+            cw.morton_order(dr.ptr, n, lo, hi, do.ptr, st)
+        assert e.value.status == 3
+    finally:
+        cw.set_workspace_limit(0)
+    # the next call is not affected and sorts again
+    dh.zero()
+    sc.trace(dr.ptr, dh.ptr, n, 0, cfg, None, st)
+    st.synchronize()
+    assert_bitwise(dh.download((n, 4)), want, "sorted launch after the limit is lifted")
+
+
+def test_per_thread_stream_handle_refused(cw, c0):
+    """The per-thread stream handle names a different stream on every host
+    thread; stream-keyed counters and workspaces cannot follow it, so traces
+    on it are refused instead of sharing state between threads."""
+    wl, lays = c0
+    sc = cw.Scene(lays["cwbvh"])
+    dr, dh = cw.DeviceBuffer.from_numpy(wl.rays[:4096]), cw.DeviceBuffer(16 * 4096)
+    per_thread = 2  # hipStreamPerThread
+    rc = cw.lib.cwbvh_trace(sc.handle, dr.ptr, dh.ptr, 4096, 0, None, None, per_thread)
+    assert rc == 1 and "hipStreamPerThread" in cw.lib.cwbvh_last_error().decode()
+
+
 def test_non_finite_vertices(cw, orc):
     """A mesh with NaN, +inf and -inf corners: the triangles they touch stay
     out of the tree, and both kernels return the oracle's hits bit for bit,
