@@ -229,6 +229,7 @@ _SIGS = {
     "cwbvh_event_synchronize": (_I32, [_P]),
     "cwbvh_event_elapsed_ms": (_I32, [_P, _P, C.POINTER(_F)]),
     "cwbvh_copy_u4": (_I32, [_P, _P, _I64, _P]),
+    "cwbvh_valu_probe": (_I32, [_I32, _I64, _I32, _P, _P]),
     "cwbvh_scene_create": (_I32, [_P, _P, _PP]),
     "cwbvh_scene_create_ex": (_I32, [_P, _P, C.c_int32, _PP]),
     "cwbvh_scene_node_stride": (_I32, [_P, C.POINTER(C.c_int32)]),

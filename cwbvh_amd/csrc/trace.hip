@@ -720,6 +720,63 @@ __global__ void copy_u4_kernel(uint4* __restrict__ dst, const uint4* __restrict_
     for (; i < n; i += step) dst[i] = src[i];
 }
 
+// VALU issue-rate probes: `iters` iterations of a fixed vector body on
+// independent chains, no memory access but one store per lane at the end, so
+// a full grid keeps every SIMD issuing vector instructions back to back.
+// Timing one launch against its SQ_INSTS_VALU count gives the issue ceiling
+// of that instruction mix:
+//   kValuProbeFma   8 chains of FP32 fma;
+//   kValuProbeInt   8 chains of 32-bit integer add / xor / min3 / max3;
+//   kValuProbeSlab  two 4-child quantized slab tests per iteration (the
+//                   node-visit body of cwbvh_trace: byte->float conversions,
+//                   fma, integer-ordered min3/max3, mask logic) on operands
+//                   that change every iteration.
+constexpr int kValuProbeFma = 0;
+constexpr int kValuProbeInt = 1;
+constexpr int kValuProbeSlab = 2;
+
+template <int MODE>
+__global__ __launch_bounds__(256) void valu_probe_kernel(float* __restrict__ out, long long iters, float a, float b) {
+    const unsigned tid = blockIdx.x * blockDim.x + threadIdx.x;
+    float f[8];
+    unsigned u[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        f[k] = (float)(tid + k) * 1e-7f;
+        u[k] = tid * 2654435761u + (unsigned)k * 40503u;
+    }
+    for (long long i = 0; i < iters; ++i) {
+        if constexpr (MODE == kValuProbeFma) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int k = 0; k < 8; ++k) f[k] = __builtin_fmaf(f[k], a, b);
+        } else if constexpr (MODE == kValuProbeInt) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const int x = (int)(u[k] + (unsigned)r), y = (int)(u[(k + 1) & 7] ^ 0x5bd1e995u);
+                    u[k] = (unsigned)max(max(x, y), (int)u[(k + 3) & 7]) ^ (unsigned)min(min(x, y), (int)u[(k + 5) & 7]);
+                }
+        } else {
+            const unsigned s = (unsigned)i;
+            const unsigned h0 = child_hits4(u[0] ^ s, u[1] + s, u[2] ^ s, u[3] + s, u[4] ^ s, u[5] + s, u[6] ^ s, f[0], f[1],
+                                            f[2], f[3], f[4], f[5], a, b, u[7]);
+            const unsigned h1 = child_hits4(u[7] ^ s, u[6] + s, u[5] ^ s, u[4] + s, u[3] ^ s, u[2] + s, u[1] ^ s, f[2], f[0],
+                                            f[1], f[5], f[3], f[4], a, b, u[0]);
+            u[0] ^= h0;
+            u[7] += h1;
+            f[0] = __builtin_fmaf(f[0], a, b);
+            f[3] = __builtin_fmaf(f[3], a, b);
+        }
+    }
+    float acc = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc += f[k] + (float)u[k];
+    out[tid] = acc;
+}
+
 // ------------------------------------------------------------ occupancy
 
 template <class K>
@@ -1007,6 +1064,21 @@ extern "C" int32_t cwbvh_copy_u4(void* dst, const void* src, int64_t n_u4, void*
     if (grid > 65536) grid = 65536;
     hipLaunchKernelGGL(copy_u4_kernel, dim3((unsigned)grid), dim3(block), 0, (hipStream_t)stream, (uint4*)dst,
                        (const uint4*)src, (long long)n_u4);
+    CW_HIP(hipGetLastError());
+    return CWBVH_OK;
+}
+
+extern "C" int32_t cwbvh_valu_probe(int32_t mode, int64_t iters, int32_t blocks, float* d_out, void* stream) {
+    CW_CHECK_ARG(mode >= kValuProbeFma && mode <= kValuProbeSlab, "mode must be 0, 1 or 2");
+    CW_CHECK_ARG(iters >= 1 && blocks >= 1 && blocks <= (1 << 20) && d_out != nullptr, "bad probe arguments");
+    const dim3 grid((unsigned)blocks), block(256);
+    hipStream_t st = (hipStream_t)stream;
+    if (mode == kValuProbeFma)
+        hipLaunchKernelGGL(valu_probe_kernel<kValuProbeFma>, grid, block, 0, st, d_out, (long long)iters, 0.999f, 1e-7f);
+    else if (mode == kValuProbeInt)
+        hipLaunchKernelGGL(valu_probe_kernel<kValuProbeInt>, grid, block, 0, st, d_out, (long long)iters, 0.999f, 1e-7f);
+    else
+        hipLaunchKernelGGL(valu_probe_kernel<kValuProbeSlab>, grid, block, 0, st, d_out, (long long)iters, 0.999f, 1e-7f);
     CW_HIP(hipGetLastError());
     return CWBVH_OK;
 }

@@ -291,6 +291,14 @@ int32_t cwbvh_event_synchronize(void* event);
 int32_t cwbvh_event_elapsed_ms(void* start, void* stop, float* ms);
 /* Calibration kernel: dst[i] = src[i] for n uint4 words (HBM copy roofline). */
 int32_t cwbvh_copy_u4(void* dst, const void* src, int64_t n_u4, void* stream);
+/* Calibration kernel for the VALU issue ceiling: `blocks` blocks of 256
+ * threads run `iters` iterations of a fixed vector body on independent
+ * register chains and store one float per thread to d_out (blocks * 256
+ * floats).  mode 0: FP32 fma; 1: 32-bit integer add/xor/min3/max3; 2: the
+ * CWBVH node-visit slab test (two 4-child quantized tests per iteration).
+ * One launch's duration against its SQ_INSTS_VALU count is the issue rate of
+ * that instruction mix with every SIMD busy. */
+int32_t cwbvh_valu_probe(int32_t mode, int64_t iters, int32_t blocks, float* d_out, void* stream);
 
 /* ------------------------------------------------- device scenes --- */
 

@@ -31,16 +31,47 @@ CFLAGS = ["-O2", "-std=c11", "-fPIC", "-shared", "-pthread", "-mfma", "-ffp-cont
 
 def build() -> Path:
     """Compile oracle.c -> liboracle.so (always rebuilds)."""
-    tmp = LIB_PATH.with_suffix(".so.tmp")
-    cmd = ["gcc", *CFLAGS, str(_DIR / "oracle.c"), "-o", str(tmp), "-lm"]
+    return _compile(LIB_PATH, CFLAGS)
+
+
+def _compile(out: Path, flags) -> Path:
+    out.parent.mkdir(exist_ok=True)
+    tmp = out.with_suffix(".so.tmp")
+    cmd = ["gcc", *flags, str(_DIR / "oracle.c"), "-o", str(tmp), "-lm"]
     proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if proc.returncode != 0:
         raise RuntimeError("oracle build failed:\n" + " ".join(cmd) + "\n" + proc.stdout)
-    os.replace(tmp, LIB_PATH)
-    return LIB_PATH
+    os.replace(tmp, out)
+    return out
+
+
+# The same restatement compiled for speed on the machine that runs it (the
+# CPU baseline of bench.py): -O3 and the host's own instruction set.  FMA
+# contraction stays off, so it returns the checker's bits.
+NATIVE_PATH = _DIR / "_native" / "liboracle_native.so"
+NATIVE_CFLAGS = ["-O3", "-march=native", "-std=c11", "-fPIC", "-shared", "-pthread", "-ffp-contract=off",
+                 "-fno-fast-math", "-Wall"]
+
+
+def _bind(L: C.CDLL) -> C.CDLL:
+    P, I64, I32 = C.c_void_p, C.c_int64, C.c_int
+    L.oracle_woop_test.argtypes = [P, P, C.c_float, C.c_float, P, P, P]
+    L.oracle_woop_test.restype = I32
+    L.oracle_trace_cwbvh.argtypes = [P, I32, P, P, P, I64, I32, P, I32, I32]
+    L.oracle_trace_cwbvh.restype = I32
+    L.oracle_trace_bvh2.argtypes = [P, P, P, P, I64, I32, P, I32, I32]
+    L.oracle_trace_bvh2.restype = I32
+    L.oracle_trace_brute.argtypes = [P, P, I64, P, P, I64, I32, I32]
+    L.oracle_trace_brute.restype = I32
+    L.oracle_woopify_ref.argtypes = [P, P, P, P]
+    L.oracle_woopify_ref.restype = None
+    L.oracle_cwbvh_decode.argtypes = [P, P, P]
+    L.oracle_cwbvh_decode.restype = None
+    return L
 
 
 _lib = None
+_native = None
 
 
 def lib() -> C.CDLL:
@@ -48,22 +79,17 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             build()
-        L = C.CDLL(str(LIB_PATH))
-        P, I64, I32 = C.c_void_p, C.c_int64, C.c_int
-        L.oracle_woop_test.argtypes = [P, P, C.c_float, C.c_float, P, P, P]
-        L.oracle_woop_test.restype = I32
-        L.oracle_trace_cwbvh.argtypes = [P, I32, P, P, P, I64, I32, P, I32, I32]
-        L.oracle_trace_cwbvh.restype = I32
-        L.oracle_trace_bvh2.argtypes = [P, P, P, P, I64, I32, P, I32, I32]
-        L.oracle_trace_bvh2.restype = I32
-        L.oracle_trace_brute.argtypes = [P, P, I64, P, P, I64, I32, I32]
-        L.oracle_trace_brute.restype = I32
-        L.oracle_woopify_ref.argtypes = [P, P, P, P]
-        L.oracle_woopify_ref.restype = None
-        L.oracle_cwbvh_decode.argtypes = [P, P, P]
-        L.oracle_cwbvh_decode.restype = None
-        _lib = L
+        _lib = _bind(C.CDLL(str(LIB_PATH)))
     return _lib
+
+
+def native_lib() -> C.CDLL:
+    """The -O3 -march=native build, compiled on first use on this host."""
+    global _native
+    if _native is None:
+        _compile(NATIVE_PATH, NATIVE_CFLAGS)
+        _native = _bind(C.CDLL(str(NATIVE_PATH)))
+    return _native
 
 
 def _f32(a, cols):
@@ -71,15 +97,22 @@ def _f32(a, cols):
 
 
 def default_threads() -> int:
-    return max(1, min(16, os.cpu_count() or 1))
+    """Host threads for the oracle: the CPUs this process may run on, at most 16."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
 
 
-def trace_cwbvh(nodes, node_stride, woop, rays, any_hit=False, threads=None, stack_capacity=64, counters=False):
-    """Scalar CWBVH traversal; returns hits (n,4) float32 [, counters (2,) uint64]."""
+def trace_cwbvh(nodes, node_stride, woop, rays, any_hit=False, threads=None, stack_capacity=64, counters=False,
+                native=False):
+    """Scalar CWBVH traversal; returns hits (n,4) float32 [, counters (2,) uint64].
+    native=True runs the -O3 -march=native build (same bits)."""
     nodes, woop, rays = _f32(nodes, 4), _f32(woop, 4), _f32(rays, 8)
     hits = np.zeros((len(rays), 4), np.float32)
     cnt = np.zeros(2, np.uint64)
-    rc = lib().oracle_trace_cwbvh(nodes.ctypes.data, node_stride, woop.ctypes.data, rays.ctypes.data,
+    rc = (native_lib() if native else lib()).oracle_trace_cwbvh(nodes.ctypes.data, node_stride, woop.ctypes.data, rays.ctypes.data,
                                   hits.ctypes.data, len(rays), 1 if any_hit else 0, cnt.ctypes.data,
                                   threads or default_threads(), stack_capacity)
     if rc != 0:
@@ -87,11 +120,11 @@ def trace_cwbvh(nodes, node_stride, woop, rays, any_hit=False, threads=None, sta
     return (hits, cnt) if counters else hits
 
 
-def trace_bvh2(nodes, woop, rays, any_hit=False, threads=None, stack_capacity=64, counters=False):
+def trace_bvh2(nodes, woop, rays, any_hit=False, threads=None, stack_capacity=64, counters=False, native=False):
     nodes, woop, rays = _f32(nodes, 4), _f32(woop, 4), _f32(rays, 8)
     hits = np.zeros((len(rays), 4), np.float32)
     cnt = np.zeros(2, np.uint64)
-    rc = lib().oracle_trace_bvh2(nodes.ctypes.data, woop.ctypes.data, rays.ctypes.data, hits.ctypes.data,
+    rc = (native_lib() if native else lib()).oracle_trace_bvh2(nodes.ctypes.data, woop.ctypes.data, rays.ctypes.data, hits.ctypes.data,
                                  len(rays), 1 if any_hit else 0, cnt.ctypes.data, threads or default_threads(),
                                  stack_capacity)
     if rc != 0:

@@ -1,6 +1,7 @@
 """GPU tests of the continuous-batching server: every request's hits equal the
-same rays traced alone (bit for bit) while requests overlap on the device,
-HBM-budget admission, serial mode and the Poisson load generator."""
+CPU oracle's and the same rays traced alone (bit for bit) while requests
+overlap on the device, HBM-budget admission, serial mode and the Poisson load
+generator."""
 import sys
 
 import numpy as np
@@ -22,7 +23,7 @@ def bits(h):
 def serving_set(cw):
     from cwbvh_amd import workloads as W
 
-    scenes, pools = [], []
+    scenes, pools, lays = [], [], []
     for i in range(4):
         m = W.heightfield_of(20000 + 15000 * i, seed=300 + i) if i % 2 == 0 else W.icospheres_of(30000, seed=300 + i)
         _, lay = cw.build_layouts(m, bvh2=False)
@@ -33,16 +34,23 @@ def serving_set(cw):
         pool.array[:] = rays
         scenes.append(sc)
         pools.append(pool)
-    return scenes, pools
+        lays.append(lay)
+    return scenes, pools, lays
 
 
 def alone(cw, sc, rays):
     return sc.trace_numpy(rays, cw.TRACE_CLOSEST)
 
 
+def oracle_hits(lay, rays):
+    import oracle
+
+    return oracle.trace_cwbvh(lay.nodes, lay.node_stride, lay.woop, rays)
+
+
 @pytest.mark.parametrize("serial", [0, 1])
 def test_requests_same_alone_and_in_mix(cw, serving_set, serial):
-    scenes, pools = serving_set
+    scenes, pools, lays = serving_set
     rng = np.random.default_rng(5)
     cfg = cw.default_server_config(serial=serial, max_inflight=3, max_batch_requests=8,
                                    memory_budget=64 << 20)
@@ -59,8 +67,10 @@ def test_requests_same_alone_and_in_mix(cw, serving_set, serial):
     for rid, s, off, n, hits in reqs:
         lat = srv.wait(rid, 60.0)
         assert lat >= 0
-        ref = alone(cw, scenes[s], pools[s].array[off:off + n])
+        rays = pools[s].array[off:off + n]
+        ref = alone(cw, scenes[s], rays)
         assert np.array_equal(bits(hits.array), bits(ref)), f"request {rid} differs from its solo trace"
+        assert np.array_equal(bits(hits.array), bits(oracle_hits(lays[s], rays))), f"request {rid} vs oracle"
     st = srv.stats()
     say(f"stats {st}")
     assert st["completed"] == len(reqs)
@@ -73,7 +83,7 @@ def test_requests_same_alone_and_in_mix(cw, serving_set, serial):
 
 
 def test_budget_admission(cw, serving_set):
-    scenes, pools = serving_set
+    scenes, pools, lays = serving_set
     # budget for ~2.5 requests of 50k rays: admission must wait for completions
     budget = int(2.5 * 50000 * 48)
     srv = cw.Server(scenes, cw.default_server_config(memory_budget=budget, max_inflight=4))
@@ -89,13 +99,14 @@ def test_budget_admission(cw, serving_set):
     for rid, s, h in outs:
         srv.wait(rid)
         assert np.array_equal(bits(h.array), bits(alone(cw, scenes[s], pools[s].array[:50000])))
+        assert np.array_equal(bits(h.array), bits(oracle_hits(lays[s], pools[s].array[:50000])))
     st = srv.stats()
     assert st["peak_bytes"] <= budget and st["completed"] == 12
     srv.close()
 
 
 def test_zero_ray_request(cw, serving_set):
-    scenes, pools = serving_set
+    scenes, pools, lays = serving_set
     srv = cw.Server(scenes)
     rid = srv.submit(1, 0, 0, 0)
     assert srv.wait(rid, 5.0) == 0.0
@@ -103,7 +114,7 @@ def test_zero_ray_request(cw, serving_set):
 
 
 def test_poisson_harness(cw, serving_set):
-    scenes, pools = serving_set
+    scenes, pools, lays = serving_set
     srv = cw.Server(scenes, cw.default_server_config(memory_budget=256 << 20))
     hit_pool = cw.PinnedArray((400000, 4), np.float32)
     say("poisson 3000 req/s x 60")

@@ -1,6 +1,7 @@
 """GPU tests of AO from surfels: the device generator returns the host
 generator's rays bit for bit, and the fused generate-and-trace kernel returns
-what tracing those rays returns (hits, or per-surfel hit counts)."""
+what the CPU oracle and the plain kernel return for those rays (hits, or
+per-surfel hit counts)."""
 import sys
 
 import numpy as np
@@ -54,7 +55,7 @@ def test_device_generator_matches_host(cw, surfels):
 @pytest.mark.parametrize("kw", [dict(), dict(loop_mode=0), dict(loop_mode=2, postpone_pct=30),
                                 dict(persistent=0), dict(persistent=1, fetch_chunk=7, block_size=64),
                                 dict(occupancy=7), dict(occupancy=7, persistent=0, block_size=256)])
-def test_fused_trace_equals_trace_of_generated_rays(cw, surfels, kw):
+def test_fused_trace_equals_trace_of_generated_rays(cw, orc, surfels, kw):
     mesh, cwb, pos, nrm, rad, diag = surfels
     k = 16
     tmax = np.float32(0.05 * diag)
@@ -68,21 +69,24 @@ def test_fused_trace_equals_trace_of_generated_rays(cw, surfels, kw):
     for flags in (cw.TRACE_CLOSEST, cw.TRACE_ANY):
         say(f"fused {kw} flags={flags}")
         ref = sc.trace_numpy(rays, flags, cfg)
+        oref = orc.trace_cwbvh(cwb.nodes, cwb.node_stride, cwb.woop, rays, any_hit=bool(flags))
         sc.trace_surfels(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, len(pos), k, 0.0, tmax, 9, flags, d_hits=dh.ptr,
                          cfg=cfg)
         cw.synchronize()
         got = dh.download((n, 4), np.float32)
         if flags == cw.TRACE_ANY and kw.get("loop_mode") == 2:
             assert np.array_equal(cw.hit_ids(got) >= 0, cw.hit_ids(ref) >= 0)
+            assert np.array_equal(cw.hit_ids(got) >= 0, cw.hit_ids(oref) >= 0)
         else:
             assert np.array_equal(bits(got), bits(ref))
+            assert np.array_equal(bits(got), bits(oref)), "fused hits differ from the oracle"
         # per-surfel counts of rays that hit
         dc.zero()
         sc.trace_surfels(bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, len(pos), k, 0.0, tmax, 9, flags,
                          d_texel_hits=dc.ptr, cfg=cfg)
         cw.synchronize()
         counts = dc.download((len(pos),), np.uint32)
-        want = (cw.hit_ids(ref) >= 0).reshape(len(pos), k).sum(axis=1)
+        want = (cw.hit_ids(oref) >= 0).reshape(len(pos), k).sum(axis=1)
         assert np.array_equal(counts, want.astype(np.uint32))
         assert 0 < counts.sum() < n
 
