@@ -488,9 +488,13 @@ int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_confi
 // Rays and hits are read and written as 16-byte vectors.
 static bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
-static int32_t trace_impl(const cwbvh_scene* scene, const cwbvh_ray* d_rays, const int32_t* d_order,
-                          cwbvh_hit* d_hits, int64_t n, uint32_t flags, const cwbvh_trace_config* cfg,
-                          uint64_t* d_counters, void* stream) {
+}  // extern "C"
+
+namespace cw {
+
+int32_t prepare_trace(const cwbvh_scene* scene, const cwbvh_ray* d_rays, const int32_t* d_order, cwbvh_hit* d_hits,
+                      int64_t n, uint32_t flags, const cwbvh_trace_config* cfg, uint64_t* d_counters, void* stream,
+                      TraceLaunch* out) {
     CW_CHECK_ARG(scene != nullptr, "scene is NULL");
     CW_CHECK_ARG(n >= 0 && n <= 0x7FFFFFFF, "ray count must be in [0, 2^31)");
     CW_CHECK_ARG(n == 0 || (d_rays != nullptr && d_hits != nullptr), "NULL ray or hit buffer");
@@ -498,13 +502,15 @@ static int32_t trace_impl(const cwbvh_scene* scene, const cwbvh_ray* d_rays, con
     CW_CHECK_ARG((flags & ~(uint32_t)(CWBVH_TRACE_ANY | CWBVH_TRACE_PRIM_ID | CWBVH_TRACE_STATS)) == 0, "unknown flags");
     CW_CHECK_ARG(!(flags & CWBVH_TRACE_STATS) || d_counters != nullptr, "STATS needs a counter buffer");
     CW_CHECK_ARG(!(flags & CWBVH_TRACE_PRIM_ID) || scene->prim_idx != nullptr, "PRIM_ID needs a prim index array");
+    CW_CHECK_ARG((hipStream_t)stream != hipStreamPerThread,
+                 "hipStreamPerThread is not supported: pass a stream handle or NULL");
     cwbvh_trace_config c;
     if (cfg) c = *cfg;
     else cwbvh_trace_config_for_scene(scene, &c);
     int32_t rc = validate_config(scene->kind, c);
     if (rc) return rc;
-    if (n == 0) return CWBVH_OK;
-    TraceLaunch L{};
+    TraceLaunch& L = *out;
+    L = TraceLaunch{};
     L.scene = scene;
     L.rays = d_rays;
     L.order = d_order;
@@ -514,6 +520,21 @@ static int32_t trace_impl(const cwbvh_scene* scene, const cwbvh_ray* d_rays, con
     L.cfg = c;
     L.stats = d_counters;
     L.stream = (hipStream_t)stream;
+    return CWBVH_OK;
+}
+
+}  // namespace cw
+
+extern "C" {
+
+static int32_t trace_impl(const cwbvh_scene* scene, const cwbvh_ray* d_rays, const int32_t* d_order,
+                          cwbvh_hit* d_hits, int64_t n, uint32_t flags, const cwbvh_trace_config* cfg,
+                          uint64_t* d_counters, void* stream) {
+    TraceLaunch L{};
+    int32_t rc = prepare_trace(scene, d_rays, d_order, d_hits, n, flags, cfg, d_counters, stream, &L);
+    if (rc) return rc;
+    if (n == 0) return CWBVH_OK;
+    const cwbvh_trace_config& c = L.cfg;
     rc = stream_counters(L.stream, &L.counters);
     if (rc) return rc;
     if (c.ray_order == 1 && d_order == nullptr && n >= kMortonMinRays) {

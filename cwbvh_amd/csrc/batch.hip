@@ -9,7 +9,6 @@
 // trace.hip), so batches of very uneven size share one launch without tails
 // per scene.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <map>
@@ -41,12 +40,11 @@ __device__ __forceinline__ unsigned morton30(float3 p, float3 lo, float3 scale) 
 }
 
 __global__ void morton_keys_single(const Ray4* __restrict__ rays, int n, float3 lo, float3 scale,
-                                   unsigned* __restrict__ keys, int* __restrict__ vals) {
+                                   unsigned* __restrict__ keys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 o = rays[i].o;
     keys[i] = morton30(make_float3(o.x, o.y, o.z), lo, scale);
-    vals[i] = i;
 }
 
 struct MortonJob {
@@ -104,10 +102,11 @@ struct cwbvh_batch {
     uint32_t flags = 0;
     cwbvh_trace_config cfg{};
     int32_t morton = 0;
-    // Morton pre-pass buffers
+    // Morton pre-pass buffers: keys (two), values (the keys kernel's job-local
+    // ray indices, a ping-pong buffer, the sorted order) and the sort's counts
     MortonJob* d_mjobs = nullptr;
     unsigned long long* d_keys[2] = {nullptr, nullptr};
-    int32_t* d_vals[2] = {nullptr, nullptr};
+    int32_t* d_vals[3] = {nullptr, nullptr, nullptr};
     void* d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int key_bits = 30;
@@ -160,35 +159,25 @@ int32_t stream_scratch(hipStream_t st, size_t bytes, char** out) {
 }
 
 // Origin Morton order of n rays on stream st, in the stream's workspace
-// (g_scratch_mu held by the caller): keys, sorted keys, ray indices, sort
-// temporaries, and, when d_order is null, the order itself (*in_scratch).
+// (g_scratch_mu held by the caller): keys, the other key buffer, two value
+// buffers, the sort's counts, and, when d_order is null, the order itself
+// (*in_scratch).
 int32_t morton_sort(const void* d_rays, int64_t n, const float* lo3, const float* hi3, hipStream_t st,
                     int32_t* d_order, int32_t** in_scratch) {
-    size_t tmp_bytes = 0;
-    unsigned* null_keys = nullptr;
-    int32_t* null_vals = nullptr;
-    CW_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, null_keys, null_keys, null_vals, null_vals, (int)n, 0,
-                                              30, st));
     const size_t col = align256((size_t)n * 4);
-    const int cols = d_order ? 3 : 4;
-    const size_t need = cols * col + align256(std::max<size_t>(tmp_bytes, 16));
+    const int cols = d_order ? 4 : 5;
+    const size_t sort_ws = align256(cw::radix_sort_workspace(n));
+    const size_t need = cols * col + sort_ws;
     char* base = nullptr;
     int32_t rc = stream_scratch(st, need, &base);
     if (rc) return rc;
-    auto* k0 = reinterpret_cast<unsigned*>(base);
-    auto* k1 = reinterpret_cast<unsigned*>(base + col);
-    auto* v0 = reinterpret_cast<int32_t*>(base + 2 * col);
     if (!d_order) {
-        d_order = reinterpret_cast<int32_t*>(base + 3 * col);
+        d_order = reinterpret_cast<int32_t*>(base + 4 * col);
         *in_scratch = d_order;
     }
-    void* tmp = base + cols * col;
-    const int block = 256;
-    hipLaunchKernelGGL(morton_keys_single, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0, st,
-                       (const Ray4*)d_rays, (int)n, make_float3(lo3[0], lo3[1], lo3[2]), scale_of(lo3, hi3), k0, v0);
-    CW_HIP(hipGetLastError());
-    CW_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, tmp_bytes, k0, k1, v0, d_order, (int)n, 0, 30, st));
-    return CWBVH_OK;
+    return cw::morton_order_into(d_rays, n, lo3, hi3, reinterpret_cast<uint32_t*>(base),
+                                 reinterpret_cast<uint32_t*>(base + col), reinterpret_cast<int32_t*>(base + 2 * col),
+                                 reinterpret_cast<int32_t*>(base + 3 * col), d_order, base + cols * col, sort_ws, st);
 }
 
 int32_t batch_sort(cwbvh_batch* b, hipStream_t st) {
@@ -198,10 +187,9 @@ int32_t batch_sort(cwbvh_batch* b, hipStream_t st) {
     hipLaunchKernelGGL(morton_keys_batch, dim3((unsigned)grid), dim3(block), 0, st, b->d_mjobs, (int)b->jobs.size(),
                        (long long)b->total_rays, b->d_keys[0], b->d_vals[0]);
     CW_HIP(hipGetLastError());
-    size_t tmp = b->sort_tmp_bytes;
-    CW_HIP(hipcub::DeviceRadixSort::SortPairs(b->d_sort_tmp, tmp, b->d_keys[0], b->d_keys[1], b->d_vals[0],
-                                              b->d_vals[1], (int)b->total_rays, 0, b->key_bits, st));
-    return CWBVH_OK;
+    // values start as the job-local ray indices the keys kernel wrote
+    return cw::radix_sort_u64(b->d_sort_tmp, b->sort_tmp_bytes, (uint64_t*)b->d_keys[0], (uint64_t*)b->d_keys[1],
+                              b->d_vals[0], b->d_vals[2], b->d_vals[0], b->d_vals[1], b->total_rays, b->key_bits, st);
 }
 
 int32_t batch_trace(cwbvh_batch* b, hipStream_t st) {
@@ -223,6 +211,17 @@ int32_t batch_trace(cwbvh_batch* b, hipStream_t st) {
 using namespace cw;
 
 namespace cw {
+
+int32_t morton_order_into(const void* d_rays, int64_t n, const float lo3[3], const float hi3[3], uint32_t* k0,
+                          uint32_t* k1, int32_t* va, int32_t* vb, int32_t* order, void* ws, size_t ws_bytes,
+                          hipStream_t st) {
+    if (n <= 0) return CWBVH_OK;
+    const int block = 256;
+    hipLaunchKernelGGL(morton_keys_single, dim3((unsigned)((n + block - 1) / block)), dim3(block), 0, st,
+                       (const Ray4*)d_rays, (int)n, make_float3(lo3[0], lo3[1], lo3[2]), scale_of(lo3, hi3), k0);
+    CW_HIP(hipGetLastError());
+    return radix_sort_u32(ws, ws_bytes, k0, k1, nullptr, va, vb, order, n, 30, st);
+}
 
 std::unique_lock<std::mutex> lock_stream_scratch() { return std::unique_lock<std::mutex>(g_scratch_mu); }
 
@@ -363,15 +362,11 @@ int32_t cwbvh_batch_create(const cwbvh_batch_item* items, int32_t count, uint32_
         b->key_bits = 30 + bits_for((int)b->jobs.size());
         e = hipMalloc(&b->d_mjobs, mj.size() * sizeof(MortonJob));
         if (e == hipSuccess) e = hipMemcpy(b->d_mjobs, mj.data(), mj.size() * sizeof(MortonJob), hipMemcpyHostToDevice);
-        for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-            e = hipMalloc(&b->d_keys[k], n * sizeof(unsigned long long));
-            if (e == hipSuccess) e = hipMalloc(&b->d_vals[k], n * sizeof(int32_t));
-        }
-        if (e == hipSuccess)
-            e = hipcub::DeviceRadixSort::SortPairs(nullptr, b->sort_tmp_bytes, b->d_keys[0], b->d_keys[1], b->d_vals[0],
-                                                   b->d_vals[1], (int)n, 0, b->key_bits, (hipStream_t)0);
-        if (e == hipSuccess) e = hipMalloc(&b->d_sort_tmp, std::max<size_t>(b->sort_tmp_bytes, 16));
-        b->bytes += (int64_t)(mj.size() * sizeof(MortonJob) + n * 24 + b->sort_tmp_bytes);
+        for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMalloc(&b->d_keys[k], n * sizeof(unsigned long long));
+        for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMalloc(&b->d_vals[k], n * sizeof(int32_t));
+        b->sort_tmp_bytes = radix_sort_workspace((int64_t)n);
+        if (e == hipSuccess) e = hipMalloc(&b->d_sort_tmp, b->sort_tmp_bytes);
+        b->bytes += (int64_t)(mj.size() * sizeof(MortonJob) + n * 28 + b->sort_tmp_bytes);
         // job j reads its rays in the order stored in its slice of the sorted values
         for (size_t j = 0; j < b->jobs.size(); ++j) b->jobs[j].order = b->d_vals[1] + b->ray_first[j];
     }
@@ -449,10 +444,10 @@ void cwbvh_batch_destroy(cwbvh_batch* b) {
     if (b->d_jobs) (void)hipFree(b->d_jobs);
     if (b->counters) (void)hipFree(b->counters);
     if (b->d_mjobs) (void)hipFree(b->d_mjobs);
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < 2; ++k)
         if (b->d_keys[k]) (void)hipFree(b->d_keys[k]);
+    for (int k = 0; k < 3; ++k)
         if (b->d_vals[k]) (void)hipFree(b->d_vals[k]);
-    }
     if (b->d_sort_tmp) (void)hipFree(b->d_sort_tmp);
     delete b;
 }

@@ -65,6 +65,12 @@ std::unique_lock<std::mutex> lock_stream_scratch();
 // stream's workspace; *order points into it until the stream's next use.
 int32_t morton_order_scratch(const void* d_rays, int64_t n, const float lo[3], const float hi[3],
                              hipStream_t stream, const int32_t** order);
+// Origin Morton order of n rays into `order`, with caller buffers: keys k0
+// and k1 and values va and vb (n entries each) and the sort's counts ws
+// (radix_sort_workspace(n) bytes).  Stream-ordered.
+int32_t morton_order_into(const void* d_rays, int64_t n, const float lo3[3], const float hi3[3], uint32_t* k0,
+                          uint32_t* k1, int32_t* va, int32_t* vb, int32_t* order, void* ws, size_t ws_bytes,
+                          hipStream_t st);
 // cwbvh_trace_config.ray_order = 1 sorts launches of at least this many rays
 // (below it the sort's fixed cost outweighs the gain)
 constexpr int64_t kMortonMinRays = int64_t(1) << 20;
@@ -94,6 +100,11 @@ struct TraceLaunch {
     uint32_t* texel_hits = nullptr;   // per-surfel hit counts instead of hits
 };
 
+// Argument checks of cwbvh_trace / cwbvh_trace_ordered and the launch record
+// they describe (counters not yet assigned; cfg NULL = the scene's default).
+int32_t prepare_trace(const cwbvh_scene* scene, const cwbvh_ray* d_rays, const int32_t* d_order, cwbvh_hit* d_hits,
+                      int64_t n, uint32_t flags, const cwbvh_trace_config* cfg, uint64_t* d_counters, void* stream,
+                      TraceLaunch* out);
 int32_t launch_cwbvh(const TraceLaunch& L);
 int32_t launch_cwbvh_surfels(const TraceLaunch& L);
 int32_t launch_surfel_rays(const cwbvh_float4* pos, const cwbvh_float4* nrm, const float* rad, int64_t num_texels,
@@ -144,5 +155,17 @@ constexpr int32_t kBatchChunk = 256;
 int32_t launch_cwbvh_batch(const BatchLaunch& L);
 
 int32_t validate_config(int32_t kind, const cwbvh_trace_config& c);
+
+// Stable LSD radix sort of (key, value) pairs (sort.hip), 8 bits per pass
+// over bits [0, key_bits).  k0 holds the keys and is overwritten; k1 is the
+// other key buffer.  Values start as vals_init (nullptr: the input index),
+// move through va / vb and end in dst, which receives the values in key
+// order: va != vals_init, vb may be vals_init, dst differs from va and vb.
+// ws holds radix_sort_workspace(n) bytes.  Stream-ordered.
+size_t radix_sort_workspace(int64_t n);
+int32_t radix_sort_u32(void* ws, size_t ws_bytes, uint32_t* k0, uint32_t* k1, const int32_t* vals_init, int32_t* va,
+                       int32_t* vb, int32_t* dst, int64_t n, int key_bits, hipStream_t st);
+int32_t radix_sort_u64(void* ws, size_t ws_bytes, uint64_t* k0, uint64_t* k1, const int32_t* vals_init, int32_t* va,
+                       int32_t* vb, int32_t* dst, int64_t n, int key_bits, hipStream_t st);
 
 }  // namespace cw

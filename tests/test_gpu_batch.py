@@ -193,12 +193,26 @@ def test_morton_order_reuses_its_workspace(cw):
         assert np.array_equal(order, np.argsort(_morton_codes(rays, lo, hi), kind="stable"))
 
     st = cw.Stream()
-    for n in (50_000, 1_000, 300_000, 7):
+    # sizes around the sort's 4096-key tiles and 64-key waves
+    for n in (50_000, 1_000, 300_000, 7, 1, 64, 65, 4096, 4097, 8191, (1 << 20) + 5):
         check(n, st)
     del st
     gc.collect()
     check(20_000, None)
     check(60_000, None)
+
+
+def test_morton_order_equal_keys_keep_ray_order(cw):
+    """Every origin the same: the stable sort returns the identity order
+    (ties keep ray order across tiles, waves and lanes)."""
+    for n in (5000, 3 * 4096 + 17):
+        rays = np.zeros((n, 8), np.float32)
+        rays[:, :3] = 0.25
+        rays[:, 4], rays[:, 7] = 1.0, np.inf
+        dr, do = cw.DeviceBuffer.from_numpy(rays), cw.DeviceBuffer(4 * n)
+        cw.morton_order(dr.ptr, n, [0, 0, 0], [1, 1, 1], do.ptr)
+        cw.synchronize()
+        assert np.array_equal(do.download((n,), np.int32), np.arange(n, dtype=np.int32))
 
 
 def test_autotune_small(cw, batch_set):
