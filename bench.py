@@ -783,17 +783,47 @@ def bench_c4(args, cw, W, dev) -> dict:
     log(f"C4: {len(scenes)} resident scenes, {tris} triangles, {hbm / 2**30:.2f} GiB of BVH in HBM, "
         f"build {build_s:.1f} s")
     hit_pool = cw.PinnedArray((16 << 20, 4), np.float32)
+    # device-resident copies of the ray pools and a device hit ring, for the
+    # requests that keep their buffers on the GPU
+    d_pools = [cw.DeviceBuffer.from_numpy(p.array) for p in pools]
+    d_hits = cw.DeviceBuffer(16 * (16 << 20))
+    # surfel pools: the pools' ray origins and directions as surfel positions
+    # and normals (8 rays per surfel), radius 2e-4 of the unit scene scale
+    rpt = 8
+    surf = []
+    for p in pools:
+        pts = p.array[::rpt]
+        sp = cw.PinnedArray((len(pts), 4), np.float32)
+        sp.array[:] = 0
+        sp.array[:, :3] = pts[:, :3]
+        sn = cw.PinnedArray((len(pts), 4), np.float32)
+        sn.array[:] = 0
+        sn.array[:, :3] = pts[:, 4:7]
+        sr = cw.PinnedArray((len(pts),), np.float32)
+        sr.array[:] = np.float32(2e-4)
+        surf.append((sp, sn, sr))
+    counts = cw.PinnedArray((1 << 21,), np.uint32)
     rates = [float(r) for r in args.c4_rates.split(",")]
     table = []
-    for serial in (0, 1):
+    modes = (("host", 0), ("host", 1), ("device", 0), ("device", 1), ("surfels", 0))
+    for kind, serial in modes:
         cfg = cw.default_server_config(serial=serial, memory_budget=8 << 30, max_inflight=3,
                                        max_batch_requests=64, max_batch_rays=16 << 20)
         srv = cw.Server(scenes, cfg)
-        srv.run_poisson(2000.0, 20, 1000, 1_000_000, pools, hit_pool, seed=1)  # warm-up
+
+        def run(rate, num, seed):
+            if kind == "device":
+                return srv.run_poisson_device(rate, num, 1000, 1_000_000, [d.ptr for d in d_pools],
+                                              [len(p.array) for p in pools], d_hits.ptr, 16 << 20, seed=seed)
+            if kind == "surfels":
+                return srv.run_poisson_surfels(rate, num, 1000, 1_000_000, surf, rpt, 0.0, 0.05, counts, seed=seed)
+            return srv.run_poisson(rate, num, 1000, 1_000_000, pools, hit_pool, seed=seed)
+
+        run(2000.0, 20, 1)  # warm-up
         for rate in rates:
-            res, _ = srv.run_poisson(rate, args.c4_requests, 1000, 1_000_000, pools, hit_pool, seed=int(rate))
+            res, _ = run(rate, args.c4_requests, int(rate))
             st = srv.stats()
-            row = {"mode": "serial" if serial else "continuous", "rate_req_s": rate,
+            row = {"requests": kind, "mode": "serial" if serial else "continuous", "rate_req_s": rate,
                    "p50_ms": round(res["p50_ms"], 3), "p99_ms": round(res["p99_ms"], 3),
                    "mean_ms": round(res["mean_ms"], 3), "mrays_per_s": round(res["mrays_per_s"], 1),
                    "wall_s": round(res["wall_s"], 3), "max_inflight_seen": st["max_inflight_seen"],
@@ -821,20 +851,33 @@ def bench_c4(args, cw, W, dev) -> dict:
         checked += nr
     srv.close()
     log(f"parity C4: {mism} of {checked} served rays differ from the oracle")
-    cont = [r for r in table if r["mode"] == "continuous"]
-    ser = [r for r in table if r["mode"] == "serial"]
+    cont = [r for r in table if r["mode"] == "continuous" and r["requests"] == "host"]
+    ser = [r for r in table if r["mode"] == "serial" and r["requests"] == "host"]
+
+    def peak(kind, mode):
+        rows = [r["mrays_per_s"] for r in table if r["requests"] == kind and r["mode"] == mode]
+        return max(rows) if rows else None
     args.steps = args.c4_requests
     args.warmup = 20
     out = base_line(METRIC + " [C4 serving]", cont[0]["p99_ms"], "ms (p99 latency at the lowest rate)", args,
                     cont[0]["mean_ms"],
                     {"workload": f"C4: {len(scenes)} resident scenes ({tris} triangles), Poisson requests of "
-                                 f"log-uniform 1k..1M AO rays, host rays in / hits out (PCIe included)",
+                                 f"log-uniform 1k..1M AO rays; value: host rays in / hits out (PCIe included)",
                      "triangles": tris, "scenes": len(scenes), "requests_per_rate": args.c4_requests,
                      "rates": rates, "seed": W.SEEDS["C4"]}, higher=False)
     out.update({
         "table": table,
-        "max_mrays_per_s": {"continuous": max(r["mrays_per_s"] for r in cont),
-                            "serial": max(r["mrays_per_s"] for r in ser)},
+        "max_mrays_per_s": {f"{k}_{m}": peak(k, m) for k, m in (("host", "continuous"), ("host", "serial"),
+                                                                 ("device", "continuous"), ("device", "serial"),
+                                                                 ("surfels", "continuous"))},
+        "p99_ms_by_rate": {f"{k}_{m}": {int(r["rate_req_s"]): r["p99_ms"] for r in table
+                                         if r["requests"] == k and r["mode"] == m}
+                           for k, m in (("host", "continuous"), ("host", "serial"), ("device", "continuous"),
+                                        ("device", "serial"), ("surfels", "continuous"))},
+        "request_kinds": {"host": "rays copied in, hits copied out (48 B per ray over PCIe)",
+                          "device": "rays and hits in device memory, used in place (no copies)",
+                          "surfels": f"surfels copied in (36 B), {rpt} AO rays generated per surfel in the traversal "
+                                     f"kernel, one 4 B hit count per surfel copied out"},
         "parity": {"rays": checked, "requests": len(reqs), "mismatches": mism, "ok": mism == 0,
                    "what": "24 overlapping served requests, every hit, closest, bitwise"},
         "bvh_hbm_gib": round(hbm / 2**30, 2),

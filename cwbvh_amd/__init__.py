@@ -142,6 +142,19 @@ class PoissonSpec(C.Structure):
                 ("max_rays", C.c_int64), ("seed", C.c_uint64)]
 
 
+class SurfelRequest(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("normals", C.c_void_p), ("radius", C.c_void_p), ("num_texels", C.c_int64),
+                ("rays_per_texel", C.c_int32), ("tmin", C.c_float), ("tmax", C.c_float), ("seed", C.c_uint64),
+                ("texel_hits", C.c_void_p)]
+
+
+class SurfelPool(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("normals", C.c_void_p), ("radius", C.c_void_p), ("num_texels", C.c_int64)]
+
+
+REQUEST_HOST_RAYS, REQUEST_DEVICE_RAYS, REQUEST_SURFELS = 0, 1, 2
+
+
 class PoissonResult(C.Structure):
     _fields_ = [("wall_s", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
                 ("max_ms", C.c_double), ("rays", C.c_int64), ("mrays_per_s", C.c_double)]
@@ -263,6 +276,10 @@ _SIGS = {
     "cwbvh_server_get_stats": (_I32, [_P, _P]),
     "cwbvh_server_destroy": (None, [_P]),
     "cwbvh_server_run_poisson": (_I32, [_P, _P, _P, _P, _P, _I64, _P, _P]),
+    "cwbvh_server_submit_device": (_I32, [_P, _I32, _P, _I64, _P, C.POINTER(_U64)]),
+    "cwbvh_server_submit_surfels": (_I32, [_P, _I32, _P, C.POINTER(_U64)]),
+    "cwbvh_server_run_poisson_device": (_I32, [_P, _P, _P, _P, _P, _I64, _P, _P]),
+    "cwbvh_server_run_poisson_surfels": (_I32, [_P, _P, _P, _I32, _F, _F, _P, _I64, _P, _P]),
     "rtBindCWBVHData": (None, [_P, _P, _P]),
     "rtTraceCWBVH": (None, [_P, _P, C.c_int]),
     "rtBindBVH2Data": (None, [_P, _P, _P]),
@@ -957,6 +974,19 @@ class Server:
         check(lib.cwbvh_server_submit(self._s, scene, host_rays_ptr, n, host_hits_ptr, C.byref(rid)))
         return rid.value
 
+    def submit_device(self, scene: int, d_rays: int, n: int, d_hits: int) -> int:
+        rid = C.c_uint64()
+        check(lib.cwbvh_server_submit_device(self._s, scene, d_rays, n, d_hits, C.byref(rid)))
+        return rid.value
+
+    def submit_surfels(self, scene: int, pos_ptr: int, nrm_ptr: int, rad_ptr: int, num_texels: int,
+                       rays_per_texel: int, tmin: float, tmax: float, seed: int, counts_ptr: int) -> int:
+        """Host (pinned) surfel arrays in, per-surfel hit counts (uint32) out."""
+        q = SurfelRequest(pos_ptr, nrm_ptr, rad_ptr, num_texels, rays_per_texel, tmin, tmax, seed, counts_ptr)
+        rid = C.c_uint64()
+        check(lib.cwbvh_server_submit_surfels(self._s, scene, C.byref(q), C.byref(rid)))
+        return rid.value
+
     def wait(self, rid: int, timeout_s: float = -1.0) -> float:
         lat = C.c_double()
         check(lib.cwbvh_server_wait(self._s, rid, timeout_s, C.byref(lat)))
@@ -980,6 +1010,30 @@ class Server:
         res = PoissonResult()
         check(lib.cwbvh_server_run_poisson(self._s, C.byref(spec), pools, sizes, hit_pool.ptr, len(hit_pool.array),
                                            lat.ctypes.data, C.byref(res)))
+        return res.as_dict(), lat
+
+    def run_poisson_device(self, rate: float, num_requests: int, min_rays: int, max_rays: int, d_ray_pools,
+                           pool_sizes, d_hit_pool: int, hit_pool_size: int, seed: int = 0):
+        """d_ray_pools: device pointers (one ray pool per scene), pool_sizes: rays per pool."""
+        spec = PoissonSpec(rate, num_requests, min_rays, max_rays, seed)
+        pools = (C.c_void_p * len(d_ray_pools))(*d_ray_pools)
+        sizes = (C.c_int64 * len(pool_sizes))(*pool_sizes)
+        lat = np.zeros(num_requests, np.float64)
+        res = PoissonResult()
+        check(lib.cwbvh_server_run_poisson_device(self._s, C.byref(spec), pools, sizes, d_hit_pool, hit_pool_size,
+                                                  lat.ctypes.data, C.byref(res)))
+        return res.as_dict(), lat
+
+    def run_poisson_surfels(self, rate: float, num_requests: int, min_rays: int, max_rays: int, pools,
+                            rays_per_texel: int, tmin: float, tmax: float, count_pool, seed: int = 0):
+        """pools: per scene (positions, normals, radius) PinnedArrays; count_pool: PinnedArray uint32."""
+        spec = PoissonSpec(rate, num_requests, min_rays, max_rays, seed)
+        arr = (SurfelPool * len(pools))(*[SurfelPool(p.ptr, nrm.ptr, r.ptr, len(p.array)) for p, nrm, r in pools])
+        lat = np.zeros(num_requests, np.float64)
+        res = PoissonResult()
+        check(lib.cwbvh_server_run_poisson_surfels(self._s, C.byref(spec), arr, rays_per_texel, tmin, tmax,
+                                                   count_pool.ptr, len(count_pool.array), lat.ctypes.data,
+                                                   C.byref(res)))
         return res.as_dict(), lat
 
     def close(self) -> None:

@@ -16,6 +16,17 @@
 // Every ray's result is independent of which batch, slot or wave traversed it
 // (deterministic tie rule), so a request returns the same hits alone and in
 // the mix.
+//
+// Three request kinds:
+//   * host rays: rays are copied into the arena and hits copied back (48 B
+//     per ray over PCIe);
+//   * device rays: the caller's device ray and hit buffers are used in place,
+//     nothing crosses PCIe and nothing is taken from the arena;
+//   * surfels: positions, normals and radii (36 B per surfel) are copied in,
+//     the fused generate-and-trace kernel makes rays_per_texel AO rays per
+//     surfel, and a 4 B count of the rays that hit comes back per surfel.
+//     A surfel request is one launch of its own on its slot (the fused
+//     kernel traverses one scene); ray requests share multi-scene launches.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -91,14 +102,28 @@ private:
 
 struct Request {
     uint64_t id = 0;
+    int32_t kind = CWBVH_REQUEST_HOST_RAYS;
     int32_t scene = 0;
-    const cwbvh_ray* host_rays = nullptr;
-    cwbvh_hit* host_hits = nullptr;
-    int64_t n = 0;
-    int64_t off = -1;        // arena offset (rays, then hits)
+    const cwbvh_ray* rays = nullptr;   // host (HOST_RAYS) or device (DEVICE_RAYS)
+    cwbvh_hit* hits = nullptr;
+    int64_t n = 0;           // rays (surfels: texels x rays per texel)
+    cwbvh_surfel_request surf{};
+    int64_t off = -1;        // arena offset (rays, then hits; or the surfel arrays, then counts)
     double t_submit = 0, t_done = 0;
     bool done = false;
     int32_t status = CWBVH_OK;
+
+    // arena bytes the request holds while in flight
+    int64_t arena_bytes() const {
+        switch (kind) {
+            case CWBVH_REQUEST_DEVICE_RAYS: return 0;
+            case CWBVH_REQUEST_SURFELS: {
+                const int64_t t = surf.num_texels;
+                return 2 * align_up(t * 16) + align_up(t * 4) + t * 4;
+            }
+            default: return align_up(n * (int64_t)sizeof(cwbvh_ray)) + n * (int64_t)sizeof(cwbvh_hit);
+        }
+    }
 };
 
 struct Slot {
@@ -141,19 +166,62 @@ struct cwbvh_server {
 };
 
 int32_t cwbvh_server::launch(Slot& sl, const std::vector<Request*>& batch) {
+    if (batch[0]->kind == CWBVH_REQUEST_SURFELS) {
+        // one fused generate-and-trace launch; per-surfel counts come back
+        const Request* r = batch[0];
+        const cwbvh_surfel_request& q = r->surf;
+        const int64_t t = q.num_texels;
+        char* base = d_arena + r->off;
+        char* pos = base;
+        char* nrm = pos + align_up(t * 16);
+        char* rad = nrm + align_up(t * 16);
+        char* cnt = rad + align_up(t * 4);
+        CW_HIP(hipMemcpyAsync(pos, q.positions, (size_t)t * 16, hipMemcpyHostToDevice, sl.stream));
+        CW_HIP(hipMemcpyAsync(nrm, q.normals, (size_t)t * 16, hipMemcpyHostToDevice, sl.stream));
+        CW_HIP(hipMemcpyAsync(rad, q.radius, (size_t)t * 4, hipMemcpyHostToDevice, sl.stream));
+        CW_HIP(hipMemsetAsync(cnt, 0, (size_t)t * 4, sl.stream));
+        TraceLaunch L{};
+        L.scene = scenes[r->scene];
+        L.n = r->n;
+        L.flags = cfg.flags & CWBVH_TRACE_ANY;
+        L.cfg = cfg.trace;
+        L.cfg.prefetch = 0;   // the fused kernel has no node prefetch
+        L.counters = sl.counters;
+        L.stream = sl.stream;
+        L.surf_pos = reinterpret_cast<const cwbvh_float4*>(pos);
+        L.surf_nrm = reinterpret_cast<const cwbvh_float4*>(nrm);
+        L.surf_rad = reinterpret_cast<const float*>(rad);
+        L.rays_per_texel = q.rays_per_texel;
+        L.gen_tmin = q.tmin;
+        L.gen_tmax = q.tmax;
+        L.seed = q.seed;
+        L.texel_hits = reinterpret_cast<uint32_t*>(cnt);
+        int32_t rc = launch_cwbvh_surfels(L);
+        if (rc) return rc;
+        CW_HIP(hipMemcpyAsync(q.texel_hits, cnt, (size_t)t * 4, hipMemcpyDeviceToHost, sl.stream));
+        CW_HIP(hipEventRecord(sl.done, sl.stream));
+        sl.reqs = batch;
+        sl.busy = true;
+        return CWBVH_OK;
+    }
     const int32_t chunk = cfg.trace.fetch_chunk;
     uint32_t chunks = 0;
     for (size_t k = 0; k < batch.size(); ++k) {
         const Request* r = batch[k];
-        char* base = d_arena + r->off;
-        CW_HIP(hipMemcpyAsync(base, r->host_rays, (size_t)r->n * sizeof(cwbvh_ray), hipMemcpyHostToDevice, sl.stream));
         const cwbvh_scene* sc = scenes[r->scene];
         BatchJob& J = sl.h_jobs[k];
+        if (r->kind == CWBVH_REQUEST_DEVICE_RAYS) {
+            J.rays = r->rays;
+            J.hits = r->hits;
+        } else {
+            char* base = d_arena + r->off;
+            CW_HIP(hipMemcpyAsync(base, r->rays, (size_t)r->n * sizeof(cwbvh_ray), hipMemcpyHostToDevice, sl.stream));
+            J.rays = base;
+            J.hits = base + align_up(r->n * (int64_t)sizeof(cwbvh_ray));
+        }
         J.nodes = sc->nodes;
         J.woop = sc->woop;
-        J.rays = base;
         J.order = nullptr;
-        J.hits = base + align_up(r->n * (int64_t)sizeof(cwbvh_ray));
         J.prim_idx = sc->prim_idx;
         J.n = (int32_t)r->n;
         J.node_stride = sc->node_stride;
@@ -190,7 +258,8 @@ int32_t cwbvh_server::launch(Slot& sl, const std::vector<Request*>& batch) {
     }
     for (size_t k = 0; k < batch.size(); ++k) {
         const Request* r = batch[k];
-        CW_HIP(hipMemcpyAsync(r->host_hits, sl.h_jobs[k].hits, (size_t)r->n * sizeof(cwbvh_hit), hipMemcpyDeviceToHost,
+        if (r->kind != CWBVH_REQUEST_HOST_RAYS) continue;
+        CW_HIP(hipMemcpyAsync(r->hits, sl.h_jobs[k].hits, (size_t)r->n * sizeof(cwbvh_hit), hipMemcpyDeviceToHost,
                               sl.stream));
     }
     CW_HIP(hipEventRecord(sl.done, sl.stream));
@@ -205,7 +274,7 @@ void cwbvh_server::complete(Slot& sl) {
     for (Request* r : sl.reqs) {
         r->t_done = t;
         r->done = true;
-        arena.release(r->off, align_up(r->n * (int64_t)sizeof(cwbvh_ray)) + r->n * (int64_t)sizeof(cwbvh_hit));
+        if (r->off >= 0) arena.release(r->off, r->arena_bytes());
         stats.completed++;
         stats.rays += r->n;
         --outstanding;
@@ -273,13 +342,19 @@ void cwbvh_server::loop() {
                 int64_t rays = 0;
                 while (!pending.empty() && (int32_t)batch.size() < max_req) {
                     Request* r = pending.front();
-                    if (!batch.empty() && rays + r->n > cfg.max_batch_rays) break;
-                    int64_t off = arena.alloc(align_up(r->n * (int64_t)sizeof(cwbvh_ray)) + r->n * (int64_t)sizeof(cwbvh_hit));
-                    if (off < 0) break;  // FIFO: wait for completions to free HBM
-                    r->off = off;
+                    const bool surfels = r->kind == CWBVH_REQUEST_SURFELS;
+                    // a surfel request is a launch of its own
+                    if (!batch.empty() && (surfels || rays + r->n > cfg.max_batch_rays)) break;
+                    const int64_t bytes = r->arena_bytes();
+                    if (bytes > 0) {
+                        const int64_t off = arena.alloc(bytes);
+                        if (off < 0) break;  // FIFO: wait for completions to free HBM
+                        r->off = off;
+                    }
                     batch.push_back(r);
                     pending.pop_front();
                     rays += r->n;
+                    if (surfels) break;
                 }
                 if (!batch.empty()) {
                     stats.peak_bytes = std::max(stats.peak_bytes, arena.used());
@@ -311,7 +386,7 @@ void cwbvh_server::loop() {
                     r->status = rc;
                     r->done = true;
                     r->t_done = t;
-                    arena.release(r->off, align_up(r->n * (int64_t)sizeof(cwbvh_ray)) + r->n * (int64_t)sizeof(cwbvh_hit));
+                    if (r->off >= 0) arena.release(r->off, r->arena_bytes());
                     --outstanding;
                 }
                 if (--inflight == 0) stats.busy_s += t - busy_since;
@@ -391,20 +466,16 @@ int32_t cwbvh_server_create(const cwbvh_scene* const* scenes, int32_t num_scenes
     return CWBVH_OK;
 }
 
-int32_t cwbvh_server_submit(cwbvh_server* s, int32_t scene, const cwbvh_ray* host_rays, int64_t n,
-                            cwbvh_hit* host_hits, uint64_t* request_id) {
-    CW_CHECK_ARG(s != nullptr && request_id != nullptr, "NULL argument");
-    CW_CHECK_ARG(scene >= 0 && scene < (int32_t)s->scenes.size(), "scene index out of range");
-    CW_CHECK_ARG(n >= 0 && n <= 0x7FFFFFFF && (n == 0 || (host_rays && host_hits)), "bad ray buffer");
-    if (align_up(n * (int64_t)sizeof(cwbvh_ray)) + n * (int64_t)sizeof(cwbvh_hit) > s->cfg.memory_budget) {
-        set_error("request of " + std::to_string(n) + " rays exceeds the serving memory budget");
+}  // extern "C"
+
+namespace {
+
+int32_t enqueue(cwbvh_server* s, Request* r, uint64_t* request_id) {
+    if (r->arena_bytes() > s->cfg.memory_budget) {
+        set_error("request of " + std::to_string(r->n) + " rays exceeds the serving memory budget");
+        delete r;
         return CWBVH_ERR_BUDGET;
     }
-    auto* r = new Request();
-    r->scene = scene;
-    r->host_rays = host_rays;
-    r->host_hits = host_hits;
-    r->n = n;
     r->t_submit = now_s();
     {
         std::lock_guard<std::mutex> g(s->mu);
@@ -415,17 +486,69 @@ int32_t cwbvh_server_submit(cwbvh_server* s, int32_t scene, const cwbvh_ray* hos
         }
         r->id = s->next_id++;
         s->requests[r->id] = r;
-        if (n == 0) {
+        if (r->n == 0) {
             r->done = true;
             r->t_done = r->t_submit;
         } else {
             s->pending.push_back(r);
             ++s->outstanding;
         }
+        *request_id = r->id;
     }
     s->cv_work.notify_one();
-    *request_id = r->id;
     return CWBVH_OK;
+}
+
+bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int32_t cwbvh_server_submit(cwbvh_server* s, int32_t scene, const cwbvh_ray* host_rays, int64_t n,
+                            cwbvh_hit* host_hits, uint64_t* request_id) {
+    CW_CHECK_ARG(s != nullptr && request_id != nullptr, "NULL argument");
+    CW_CHECK_ARG(scene >= 0 && scene < (int32_t)s->scenes.size(), "scene index out of range");
+    CW_CHECK_ARG(n >= 0 && n <= 0x7FFFFFFF && (n == 0 || (host_rays && host_hits)), "bad ray buffer");
+    auto* r = new Request();
+    r->kind = CWBVH_REQUEST_HOST_RAYS;
+    r->scene = scene;
+    r->rays = host_rays;
+    r->hits = host_hits;
+    r->n = n;
+    return enqueue(s, r, request_id);
+}
+
+int32_t cwbvh_server_submit_device(cwbvh_server* s, int32_t scene, const cwbvh_ray* d_rays, int64_t n,
+                                   cwbvh_hit* d_hits, uint64_t* request_id) {
+    CW_CHECK_ARG(s != nullptr && request_id != nullptr, "NULL argument");
+    CW_CHECK_ARG(scene >= 0 && scene < (int32_t)s->scenes.size(), "scene index out of range");
+    CW_CHECK_ARG(n >= 0 && n <= 0x7FFFFFFF && (n == 0 || (d_rays && d_hits)), "bad ray buffer");
+    CW_CHECK_ARG(aligned16(d_rays) && aligned16(d_hits), "ray and hit buffers must be 16-byte aligned");
+    auto* r = new Request();
+    r->kind = CWBVH_REQUEST_DEVICE_RAYS;
+    r->scene = scene;
+    r->rays = d_rays;
+    r->hits = d_hits;
+    r->n = n;
+    return enqueue(s, r, request_id);
+}
+
+int32_t cwbvh_server_submit_surfels(cwbvh_server* s, int32_t scene, const cwbvh_surfel_request* q,
+                                    uint64_t* request_id) {
+    CW_CHECK_ARG(s != nullptr && request_id != nullptr && q != nullptr, "NULL argument");
+    CW_CHECK_ARG(scene >= 0 && scene < (int32_t)s->scenes.size(), "scene index out of range");
+    CW_CHECK_ARG(q->num_texels >= 0 && q->rays_per_texel >= 1 &&
+                     q->num_texels * (int64_t)q->rays_per_texel <= 0x7FFFFFFF,
+                 "bad surfel count");
+    CW_CHECK_ARG(q->num_texels == 0 || (q->positions && q->normals && q->radius && q->texel_hits),
+                 "NULL surfel array");
+    auto* r = new Request();
+    r->kind = CWBVH_REQUEST_SURFELS;
+    r->scene = scene;
+    r->surf = *q;
+    r->n = q->num_texels * (int64_t)q->rays_per_texel;
+    return enqueue(s, r, request_id);
 }
 
 int32_t cwbvh_server_wait(cwbvh_server* s, uint64_t id, double timeout_s, double* latency_ms) {
@@ -484,15 +607,19 @@ void cwbvh_server_destroy(cwbvh_server* s) {
     delete s;
 }
 
-int32_t cwbvh_server_run_poisson(cwbvh_server* s, const cwbvh_poisson_spec* spec, const cwbvh_ray* const* ray_pools,
-                                 const int64_t* pool_sizes, cwbvh_hit* hit_pool, int64_t hit_pool_size,
-                                 double* latencies_ms, cwbvh_poisson_result* out) {
-    CW_CHECK_ARG(s && spec && ray_pools && pool_sizes && hit_pool && out, "NULL argument");
-    CW_CHECK_ARG(spec->rate > 0 && spec->num_requests >= 1 && spec->min_rays >= 1 && spec->max_rays >= spec->min_rays,
-                 "bad Poisson spec");
+}  // extern "C"
+
+namespace {
+
+// Poisson arrivals: request k arrives after an exponential gap, picks a scene
+// uniformly and a ray count log-uniform in [min_rays, max_rays] (at most the
+// scene's pool), and is handed to submit(k, scene, rays, pool offset, ring
+// offset) -> status, where ring offsets wrap at ring_size.  Then every
+// request is waited for and the latency percentiles are computed.
+template <class Submit>
+int32_t drive_poisson(cwbvh_server* s, const cwbvh_poisson_spec* spec, const int64_t* pool_sizes, int64_t ring_size,
+                      int64_t ray_unit, double* latencies_ms, cwbvh_poisson_result* out, Submit submit) {
     const int32_t ns = (int32_t)s->scenes.size();
-    for (int32_t k = 0; k < ns; ++k) CW_CHECK_ARG(ray_pools[k] && pool_sizes[k] >= 1, "empty ray pool");
-    CW_CHECK_ARG(hit_pool_size >= spec->max_rays, "hit pool smaller than the largest request");
     std::mt19937_64 rng(spec->seed);
     std::exponential_distribution<double> gap(spec->rate);
     std::uniform_real_distribution<double> u01(0.0, 1.0);
@@ -506,27 +633,29 @@ int32_t cwbvh_server_run_poisson(cwbvh_server* s, const cwbvh_poisson_spec* spec
     for (int32_t k = 0; k < spec->num_requests && rc == CWBVH_OK; ++k) {
         t_next += gap(rng);
         const int32_t sc = (int32_t)(u01(rng) * ns) % ns;
-        int64_t n = (int64_t)std::exp(lmin + (lmax - lmin) * u01(rng));
-        n = std::min(std::max<int64_t>(n, spec->min_rays), std::min<int64_t>(spec->max_rays, pool_sizes[sc]));
+        // request size in pool units (rays, or surfels of ray_unit rays)
+        int64_t n = (int64_t)std::exp(lmin + (lmax - lmin) * u01(rng)) / ray_unit;
+        n = std::min(std::max<int64_t>(n, std::max<int64_t>(1, spec->min_rays / ray_unit)),
+                     std::min<int64_t>(spec->max_rays / ray_unit, pool_sizes[sc]));
+        n = std::max<int64_t>(n, 1);
         const int64_t off = pool_sizes[sc] > n ? (int64_t)(u01(rng) * (double)(pool_sizes[sc] - n)) : 0;
-        if (ring + n > hit_pool_size) ring = 0;
+        if (ring + n > ring_size) ring = 0;
         while (now_s() < t_next) {
             // arrival clock: sleep when far ahead, spin for the last tens of microseconds
             const double ahead = t_next - now_s();
             if (ahead > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(ahead - 1e-4));
         }
-        rc = cwbvh_server_submit(s, sc, ray_pools[sc] + off, n, hit_pool + ring, &ids[k]);
+        rc = submit(sc, n, off, ring, &ids[k]);
         ring += n;
-        sizes[k] = n;
+        sizes[k] = n * ray_unit;
     }
     if (rc) return rc;
     std::vector<double> lat(spec->num_requests);
-    double t_last = t0;
     for (int32_t k = 0; k < spec->num_requests; ++k) {
         rc = cwbvh_server_wait(s, ids[k], -1.0, &lat[k]);
         if (rc) return rc;
     }
-    t_last = now_s();
+    const double t_last = now_s();
     int64_t rays = 0;
     for (int64_t v : sizes) rays += v;
     if (latencies_ms) std::copy(lat.begin(), lat.end(), latencies_ms);
@@ -546,6 +675,77 @@ int32_t cwbvh_server_run_poisson(cwbvh_server* s, const cwbvh_poisson_spec* spec
     out->rays = rays;
     out->mrays_per_s = rays / out->wall_s / 1e6;
     return CWBVH_OK;
+}
+
+int32_t check_spec(const cwbvh_poisson_spec* spec) {
+    CW_CHECK_ARG(spec->rate > 0 && spec->num_requests >= 1 && spec->min_rays >= 1 && spec->max_rays >= spec->min_rays,
+                 "bad Poisson spec");
+    return CWBVH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t cwbvh_server_run_poisson(cwbvh_server* s, const cwbvh_poisson_spec* spec, const cwbvh_ray* const* ray_pools,
+                                 const int64_t* pool_sizes, cwbvh_hit* hit_pool, int64_t hit_pool_size,
+                                 double* latencies_ms, cwbvh_poisson_result* out) {
+    CW_CHECK_ARG(s && spec && ray_pools && pool_sizes && hit_pool && out, "NULL argument");
+    int32_t rc = check_spec(spec);
+    if (rc) return rc;
+    for (size_t k = 0; k < s->scenes.size(); ++k) CW_CHECK_ARG(ray_pools[k] && pool_sizes[k] >= 1, "empty ray pool");
+    CW_CHECK_ARG(hit_pool_size >= spec->max_rays, "hit pool smaller than the largest request");
+    return drive_poisson(s, spec, pool_sizes, hit_pool_size, 1, latencies_ms, out,
+                         [&](int32_t sc, int64_t n, int64_t off, int64_t ring, uint64_t* id) {
+                             return cwbvh_server_submit(s, sc, ray_pools[sc] + off, n, hit_pool + ring, id);
+                         });
+}
+
+int32_t cwbvh_server_run_poisson_device(cwbvh_server* s, const cwbvh_poisson_spec* spec,
+                                        const cwbvh_ray* const* d_ray_pools, const int64_t* pool_sizes,
+                                        cwbvh_hit* d_hit_pool, int64_t hit_pool_size, double* latencies_ms,
+                                        cwbvh_poisson_result* out) {
+    CW_CHECK_ARG(s && spec && d_ray_pools && pool_sizes && d_hit_pool && out, "NULL argument");
+    int32_t rc = check_spec(spec);
+    if (rc) return rc;
+    for (size_t k = 0; k < s->scenes.size(); ++k) CW_CHECK_ARG(d_ray_pools[k] && pool_sizes[k] >= 1, "empty ray pool");
+    CW_CHECK_ARG(hit_pool_size >= spec->max_rays, "hit pool smaller than the largest request");
+    return drive_poisson(s, spec, pool_sizes, hit_pool_size, 1, latencies_ms, out,
+                         [&](int32_t sc, int64_t n, int64_t off, int64_t ring, uint64_t* id) {
+                             return cwbvh_server_submit_device(s, sc, d_ray_pools[sc] + off, n, d_hit_pool + ring, id);
+                         });
+}
+
+int32_t cwbvh_server_run_poisson_surfels(cwbvh_server* s, const cwbvh_poisson_spec* spec,
+                                         const cwbvh_surfel_pool* pools, int32_t rays_per_texel, float tmin,
+                                         float tmax, uint32_t* count_pool, int64_t count_pool_size,
+                                         double* latencies_ms, cwbvh_poisson_result* out) {
+    CW_CHECK_ARG(s && spec && pools && count_pool && out, "NULL argument");
+    int32_t rc = check_spec(spec);
+    if (rc) return rc;
+    CW_CHECK_ARG(rays_per_texel >= 1 && spec->max_rays >= rays_per_texel, "bad rays per texel");
+    const size_t ns = s->scenes.size();
+    std::vector<int64_t> sizes(ns);
+    for (size_t k = 0; k < ns; ++k) {
+        CW_CHECK_ARG(pools[k].positions && pools[k].normals && pools[k].radius && pools[k].num_texels >= 1,
+                     "empty surfel pool");
+        sizes[k] = pools[k].num_texels;
+    }
+    CW_CHECK_ARG(count_pool_size >= spec->max_rays / rays_per_texel, "count pool smaller than the largest request");
+    return drive_poisson(s, spec, sizes.data(), count_pool_size, rays_per_texel, latencies_ms, out,
+                         [&](int32_t sc, int64_t t, int64_t off, int64_t ring, uint64_t* id) {
+                             cwbvh_surfel_request q{};
+                             q.positions = pools[sc].positions + off;
+                             q.normals = pools[sc].normals + off;
+                             q.radius = pools[sc].radius + off;
+                             q.num_texels = t;
+                             q.rays_per_texel = rays_per_texel;
+                             q.tmin = tmin;
+                             q.tmax = tmax;
+                             q.seed = spec->seed + (uint64_t)off;
+                             q.texel_hits = count_pool + ring;
+                             return cwbvh_server_submit_surfels(s, sc, &q, id);
+                         });
 }
 
 }  // extern "C"

@@ -498,6 +498,41 @@ int32_t cwbvh_server_create(const cwbvh_scene* const* scenes, int32_t num_scenes
  * than the memory budget fail with CWBVH_ERR_BUDGET. */
 int32_t cwbvh_server_submit(cwbvh_server* s, int32_t scene, const cwbvh_ray* host_rays, int64_t n,
                             cwbvh_hit* host_hits, uint64_t* request_id);
+/* Request kinds (cwbvh_server_submit, _submit_device, _submit_surfels). */
+enum {
+    CWBVH_REQUEST_HOST_RAYS = 0,    /* rays copied in, hits copied out (48 B per ray over PCIe) */
+    CWBVH_REQUEST_DEVICE_RAYS = 1,  /* caller's device buffers used in place: no copies, no arena */
+    CWBVH_REQUEST_SURFELS = 2       /* surfels in, per-surfel hit counts out (40 B per surfel) */
+};
+/* Queue a request on device-resident buffers: n rays at d_rays (16-byte
+ * aligned device memory) traced against scenes[scene], hits written to
+ * d_hits.  Nothing crosses PCIe and the memory budget is not charged; both
+ * buffers must stay valid and untouched until completion.  Requests of this
+ * kind join the same multi-scene launches as host-ray requests. */
+int32_t cwbvh_server_submit_device(cwbvh_server* s, int32_t scene, const cwbvh_ray* d_rays, int64_t n,
+                                   cwbvh_hit* d_hits, uint64_t* request_id);
+/* An AO query from surfels (the reference's sample file contents,
+ * src/main.cpp:52-64): num_texels host positions, normals and radii; the
+ * server copies them into its arena (36 B per surfel, charged to the memory
+ * budget with the 4 B count), generates rays_per_texel AO rays per surfel
+ * inside the traversal kernel (as cwbvh_trace_surfels does, tmin / tmax /
+ * seed as there) and writes to texel_hits[i] the number of surfel i's rays
+ * that hit something.  Host arrays should be pinned and must stay valid until
+ * completion.  A surfel request is one fused launch of its own on a free
+ * slot; it overlaps with the other slots' launches. */
+typedef struct cwbvh_surfel_request {
+    const cwbvh_float4* positions;
+    const cwbvh_float4* normals;
+    const float* radius;
+    int64_t num_texels;
+    int32_t rays_per_texel;
+    float tmin;
+    float tmax;
+    uint64_t seed;
+    uint32_t* texel_hits;
+} cwbvh_surfel_request;
+int32_t cwbvh_server_submit_surfels(cwbvh_server* s, int32_t scene, const cwbvh_surfel_request* request,
+                                    uint64_t* request_id);
 /* Wait for a request (timeout_s < 0: no limit); *latency_ms = completion -
  * submission on the host clock. */
 int32_t cwbvh_server_wait(cwbvh_server* s, uint64_t request_id, double timeout_s,
@@ -551,6 +586,27 @@ int32_t cwbvh_server_run_poisson(cwbvh_server* s, const cwbvh_poisson_spec* spec
                                  const cwbvh_ray* const* ray_pools, const int64_t* pool_sizes,
                                  cwbvh_hit* hit_pool, int64_t hit_pool_size,
                                  double* latencies_ms, cwbvh_poisson_result* out);
+/* The same load on device-resident buffers (CWBVH_REQUEST_DEVICE_RAYS):
+ * d_ray_pools[scene] and d_hit_pool are device arrays. */
+int32_t cwbvh_server_run_poisson_device(cwbvh_server* s, const cwbvh_poisson_spec* spec,
+                                        const cwbvh_ray* const* d_ray_pools, const int64_t* pool_sizes,
+                                        cwbvh_hit* d_hit_pool, int64_t hit_pool_size, double* latencies_ms,
+                                        cwbvh_poisson_result* out);
+/* The same load as surfel requests (CWBVH_REQUEST_SURFELS): a request of r
+ * rays (log-uniform as above) asks for r / rays_per_texel surfels, a window
+ * of pools[scene] (pinned host arrays); counts land in count_pool (pinned,
+ * count_pool_size entries, used as a ring).  Request k's seed is spec->seed
+ * plus its window offset. */
+typedef struct cwbvh_surfel_pool {
+    const cwbvh_float4* positions;
+    const cwbvh_float4* normals;
+    const float* radius;
+    int64_t num_texels;
+} cwbvh_surfel_pool;
+int32_t cwbvh_server_run_poisson_surfels(cwbvh_server* s, const cwbvh_poisson_spec* spec,
+                                         const cwbvh_surfel_pool* pools, int32_t rays_per_texel, float tmin,
+                                         float tmax, uint32_t* count_pool, int64_t count_pool_size,
+                                         double* latencies_ms, cwbvh_poisson_result* out);
 
 /* ---------------------------------------------------------- autotuner --- */
 

@@ -124,3 +124,106 @@ def test_poisson_harness(cw, serving_set):
     assert np.all(lat > 0) and len(lat) == 60
     assert srv.stats()["completed"] == 60
     srv.close()
+
+
+@pytest.fixture(scope="module")
+def surfel_pools(cw, serving_set):
+    """Per scene: 20,000 surfels on the surface (pinned positions, normals,
+    radii), the window the surfel requests below take from."""
+    scenes, pools, lays = serving_set
+    rng = np.random.default_rng(71)
+    out = []
+    for k, pool in enumerate(pools):
+        pts = pool.array[::8][:20000]
+        pos = cw.PinnedArray((len(pts), 4), np.float32)
+        pos.array[:] = 0
+        pos.array[:, :3] = pts[:, :3]
+        nrm = cw.PinnedArray((len(pts), 4), np.float32)
+        nrm.array[:] = 0
+        nrm.array[:, :3] = pts[:, 4:7]
+        rad = cw.PinnedArray((len(pts),), np.float32)
+        rad.array[:] = rng.uniform(1e-4, 1e-3, size=len(pts)).astype(np.float32)
+        out.append((pos, nrm, rad))
+    return out
+
+
+def test_device_and_host_requests_in_one_mix_match_oracle(cw, serving_set):
+    """Device-resident requests (no copies) and host requests share batches;
+    every request's hits equal the oracle's."""
+    scenes, pools, lays = serving_set
+    rng = np.random.default_rng(6)
+    dpools = [cw.DeviceBuffer.from_numpy(p.array) for p in pools]
+    srv = cw.Server(scenes, cw.default_server_config(max_inflight=3, max_batch_requests=8, memory_budget=64 << 20))
+    reqs = []
+    for k in range(40):
+        s = int(rng.integers(0, len(scenes)))
+        n = int(np.exp(rng.uniform(np.log(100), np.log(60000))))
+        off = int(rng.integers(0, len(pools[s].array) - n))
+        if k % 2:
+            dh = cw.DeviceBuffer(16 * n)
+            rid = srv.submit_device(s, dpools[s].ptr + 32 * off, n, dh.ptr)
+            reqs.append((rid, s, off, n, dh, True))
+        else:
+            h = cw.PinnedArray((n, 4), np.float32)
+            reqs.append((srv.submit(s, pools[s].ptr + 32 * off, n, h.ptr), s, off, n, h, False))
+    say(f"submitted {len(reqs)} host and device requests")
+    for rid, s, off, n, buf, dev in reqs:
+        srv.wait(rid, 60.0)
+        got = buf.download((n, 4)) if dev else buf.array
+        ref = oracle_hits(lays[s], pools[s].array[off:off + n])
+        assert np.array_equal(bits(got), bits(ref)), f"request {rid} (device={dev}) differs from the oracle"
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["max_batch_requests_seen"] > 1
+    srv.close()
+
+
+@pytest.mark.parametrize("any_hit", [False, True])
+def test_surfel_requests_match_oracle(cw, serving_set, surfel_pools, any_hit):
+    """Surfel requests return, per surfel, the number of its generated AO
+    rays that hit: the oracle's count on the host generator's rays."""
+    import oracle
+
+    scenes, pools, lays = serving_set
+    rng = np.random.default_rng(8)
+    flags = cw.TRACE_ANY if any_hit else cw.TRACE_CLOSEST
+    srv = cw.Server(scenes, cw.default_server_config(max_inflight=3, memory_budget=64 << 20, flags=flags))
+    reqs = []
+    for k in range(12):
+        s = int(rng.integers(0, len(scenes)))
+        pos, nrm, rad = surfel_pools[s]
+        t = int(rng.integers(1, 4000))
+        off = int(rng.integers(0, len(pos.array) - t))
+        rpt = int(rng.integers(1, 20))
+        diag = 1.0
+        counts = cw.PinnedArray((t,), np.uint32)
+        rid = srv.submit_surfels(s, pos.ptr + 16 * off, nrm.ptr + 16 * off, rad.ptr + 4 * off, t, rpt, 0.0,
+                                 0.05 * diag, 100 + k, counts.ptr)
+        reqs.append((rid, s, off, t, rpt, 100 + k, counts))
+    say(f"submitted {len(reqs)} surfel requests any={any_hit}")
+    for rid, s, off, t, rpt, seed, counts in reqs:
+        srv.wait(rid, 60.0)
+        pos, nrm, rad = surfel_pools[s]
+        rays = cw.rays_ao_from_samples(pos.array[off:off + t], nrm.array[off:off + t], rad.array[off:off + t], rpt,
+                                       0.0, np.float32(0.05), seed)
+        ref = oracle.trace_cwbvh(lays[s].nodes, lays[s].node_stride, lays[s].woop, rays, any_hit=any_hit)
+        want = (oracle.hit_ids(ref) >= 0).reshape(t, rpt).sum(axis=1).astype(np.uint32)
+        assert np.array_equal(counts.array, want), f"surfel request {rid}: counts differ from the oracle"
+    assert srv.stats()["completed"] == len(reqs)
+    srv.close()
+
+
+def test_poisson_device_and_surfel_harnesses(cw, serving_set, surfel_pools):
+    scenes, pools, lays = serving_set
+    srv = cw.Server(scenes, cw.default_server_config(memory_budget=256 << 20))
+    dpools = [cw.DeviceBuffer.from_numpy(p.array) for p in pools]
+    dhits = cw.DeviceBuffer(16 * 400000)
+    say("poisson device 3000 req/s x 60")
+    res, lat = srv.run_poisson_device(3000.0, 60, 1000, 100000, [d.ptr for d in dpools],
+                                      [len(p.array) for p in pools], dhits.ptr, 400000, seed=9)
+    assert res["rays"] > 0 and 0 < res["p50_ms"] <= res["p99_ms"] and len(lat) == 60
+    counts = cw.PinnedArray((100000,), np.uint32)
+    say("poisson surfels 3000 req/s x 60")
+    res, lat = srv.run_poisson_surfels(3000.0, 60, 1000, 100000, surfel_pools, 8, 0.0, 0.05, counts, seed=10)
+    assert res["rays"] > 0 and 0 < res["p50_ms"] <= res["p99_ms"] and np.all(lat > 0)
+    assert srv.stats()["completed"] == 120
+    srv.close()
