@@ -166,7 +166,7 @@ int32_t morton_sort(const void* d_rays, int64_t n, const float* lo3, const float
                     int32_t* d_order, int32_t** in_scratch) {
     const size_t col = align256((size_t)n * 4);
     const int cols = d_order ? 4 : 5;
-    const size_t sort_ws = align256(cw::radix_sort_workspace(n));
+    const size_t sort_ws = align256(cw::radix_sort_workspace(n, 30));
     const size_t need = cols * col + sort_ws;
     char* base = nullptr;
     int32_t rc = stream_scratch(st, need, &base);
@@ -364,7 +364,7 @@ int32_t cwbvh_batch_create(const cwbvh_batch_item* items, int32_t count, uint32_
         if (e == hipSuccess) e = hipMemcpy(b->d_mjobs, mj.data(), mj.size() * sizeof(MortonJob), hipMemcpyHostToDevice);
         for (int k = 0; k < 2 && e == hipSuccess; ++k) e = hipMalloc(&b->d_keys[k], n * sizeof(unsigned long long));
         for (int k = 0; k < 3 && e == hipSuccess; ++k) e = hipMalloc(&b->d_vals[k], n * sizeof(int32_t));
-        b->sort_tmp_bytes = radix_sort_workspace((int64_t)n);
+        b->sort_tmp_bytes = radix_sort_workspace((int64_t)n, b->key_bits);
         if (e == hipSuccess) e = hipMalloc(&b->d_sort_tmp, b->sort_tmp_bytes);
         b->bytes += (int64_t)(mj.size() * sizeof(MortonJob) + n * 28 + b->sort_tmp_bytes);
         // job j reads its rays in the order stored in its slice of the sorted values

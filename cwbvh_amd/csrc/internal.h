@@ -67,7 +67,7 @@ int32_t morton_order_scratch(const void* d_rays, int64_t n, const float lo[3], c
                              hipStream_t stream, const int32_t** order);
 // Origin Morton order of n rays into `order`, with caller buffers: keys k0
 // and k1 and values va and vb (n entries each) and the sort's counts ws
-// (radix_sort_workspace(n) bytes).  Stream-ordered.
+// (radix_sort_workspace(n, 30) bytes).  Stream-ordered.
 int32_t morton_order_into(const void* d_rays, int64_t n, const float lo3[3], const float hi3[3], uint32_t* k0,
                           uint32_t* k1, int32_t* va, int32_t* vb, int32_t* order, void* ws, size_t ws_bytes,
                           hipStream_t st);
@@ -161,8 +161,8 @@ int32_t validate_config(int32_t kind, const cwbvh_trace_config& c);
 // other key buffer.  Values start as vals_init (nullptr: the input index),
 // move through va / vb and end in dst, which receives the values in key
 // order: va != vals_init, vb may be vals_init, dst differs from va and vb.
-// ws holds radix_sort_workspace(n) bytes.  Stream-ordered.
-size_t radix_sort_workspace(int64_t n);
+// ws holds radix_sort_workspace(n, key_bits) bytes.  Stream-ordered.
+size_t radix_sort_workspace(int64_t n, int key_bits);
 int32_t radix_sort_u32(void* ws, size_t ws_bytes, uint32_t* k0, uint32_t* k1, const int32_t* vals_init, int32_t* va,
                        int32_t* vb, int32_t* dst, int64_t n, int key_bits, hipStream_t st);
 int32_t radix_sort_u64(void* ws, size_t ws_bytes, uint64_t* k0, uint64_t* k1, const int32_t* vals_init, int32_t* va,
