@@ -273,12 +273,16 @@ def bench_single(args, cw, W, dev, name: str) -> dict:
     def run(sc, fl, cfg, dr=d_rays, dh=d_hits, cnt=n):
         return lambda: sc.trace(dr.ptr, dh.ptr, cnt, fl, cfg)
 
+    # the timed launches are a profiler region (rocprofv3 --selected-regions
+    # collects these and nothing of the set-up, counting or parity launches)
     log("timing CWBVH closest + any")
+    cw.profiler_start()
     total_ms, per = time_launches([run(s_cw, 0, cfg_cw), run(s_cw, 1, cfg_cw)], args.steps, args.warmup, cw)
     ms_step = total_ms / args.steps
     ms = {"cwbvh_closest": float(per[:, 0].mean()), "cwbvh_any": float(per[:, 1].mean())}
     log("timing BVH2 closest + any")
     _, per2 = time_launches([run(s_b2, 0, cfg_b2), run(s_b2, 1, cfg_b2)], args.steps, args.warmup, cw)
+    cw.profiler_stop()
     ms["bvh2_closest"] = float(per2[:, 0].mean())
     ms["bvh2_any"] = float(per2[:, 1].mean())
     modes = {k: n / v / 1e3 for k, v in ms.items()}
@@ -912,6 +916,8 @@ def profile_launches(args, cw, W) -> dict:
     src, dst = cw.DeviceBuffer(copy_bytes), cw.DeviceBuffer(copy_bytes)
     src.zero()
     cw.synchronize()
+    # everything below is one profiler region (rocprofv3 --selected-regions)
+    cw.profiler_start()
     cw.check(cw.lib.cwbvh_copy_u4(dst.ptr, src.ptr, copy_bytes // 16, None))
     probe_blocks = 8192
     probe = cw.DeviceBuffer(4 * 256 * probe_blocks)
@@ -923,6 +929,7 @@ def profile_launches(args, cw, W) -> dict:
     for _ in range(args.launches):
         s_b2.trace(dr.ptr, dh.ptr, n, flags, cfg_b2)
     cw.synchronize()
+    cw.profiler_stop()
     return {"workload": wl.name, "rays": n, "copy_bytes": copy_bytes, "launches": args.launches, "flags": flags,
             "probe": {"blocks": probe_blocks, "threads_per_block": 256, "iters": args.probe_iters},
             "cwbvh_config": cfg.as_dict(), "bvh2_config": cfg_b2.as_dict(), "node_bytes": 16 * s_cw.node_stride,

@@ -285,6 +285,8 @@ _SIGS = {
     "rtBindBVH2Data": (None, [_P, _P, _P]),
     "rtTraceBVH2": (None, [_P, _P, C.c_int]),
     "cwbvh_last_status": (_I32, []),
+    "cwbvh_profiler_start": (_I32, []),
+    "cwbvh_profiler_stop": (_I32, []),
     "cwbvh_last_trace_ms": (_F, []),
     "cwbvh_set_log": (None, [_I32]),
 }
@@ -860,6 +862,16 @@ def morton_order(d_rays: int, n: int, lo, hi, d_order: int, stream: Stream | Non
     lo = _f32(lo)
     hi = _f32(hi)
     check(lib.cwbvh_morton_order(d_rays, n, _ptr(lo), _ptr(hi), d_order, stream.handle if stream else None))
+
+
+def profiler_start() -> None:
+    """Resume profiler collection (rocprofv3 --selected-regions)."""
+    check(lib.cwbvh_profiler_start())
+
+
+def profiler_stop() -> None:
+    """Pause profiler collection."""
+    check(lib.cwbvh_profiler_stop())
 
 
 def set_workspace_limit(nbytes: int) -> None:

@@ -66,7 +66,7 @@ def build(verbose: bool = False) -> Path:
         objs = list(ex.map(_compile, sources))
     tmp = LIB.with_suffix(".so.tmp")
     _run([str(ROCM / "bin" / "hipcc"), "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp),
-          *map(str, objs), "-pthread"])
+          *map(str, objs), "-pthread", f"-L{ROCM / 'lib'}", "-lrocprofiler-sdk-roctx"])
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")

@@ -18,6 +18,7 @@
 //    for FFI callers that cannot bind mangled C++ names.
 #include <hip/hip_runtime_api.h>
 #include <hip/hip_vector_types.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <cstdio>
 #include <mutex>
@@ -63,10 +64,17 @@ void trace(cwbvh_scene* scene, cwbvh_ray* rays, cwbvh_hit* hits, int count, cons
         g_status = cw::hip_fail(ce, "rtTrace event");
         return;
     }
+    // the reference brackets each trace with cudaProfilerStart / Stop
+    // (src/TraversalKernelCWBVH.cu:404,423): the roctx counterparts scope
+    // rocprofv3 --selected-regions to the launch
+    roctxProfilerResume(0);
+    roctxRangePush(label);
     hipEventRecord(a, nullptr);
     int32_t rc = cwbvh_trace(scene, rays, hits, count, CWBVH_TRACE_CLOSEST, nullptr, nullptr, nullptr);
     hipEventRecord(b, nullptr);
     hipError_t e = hipEventSynchronize(b);
+    roctxRangePop();
+    roctxProfilerPause(0);
     if (rc == CWBVH_OK && e != hipSuccess) rc = cw::hip_fail(e, "rtTrace");
     float ms = 0.f;
     hipEventElapsedTime(&ms, a, b);
@@ -106,6 +114,16 @@ void rtBindBVH2Data(const cwbvh_float4* InBVHTreeNodes, const cwbvh_float4* InTr
 
 void rtTraceBVH2(cwbvh_ray* rayBuffer, cwbvh_hit* rayResultBuffer, int rayCount) {
     trace(g_bvh2, rayBuffer, rayResultBuffer, rayCount, "rtTraceBVH2 No Dynamic Fetch");
+}
+
+int32_t cwbvh_profiler_start(void) {
+    roctxProfilerResume(0);
+    return CWBVH_OK;
+}
+
+int32_t cwbvh_profiler_stop(void) {
+    roctxProfilerPause(0);
+    return CWBVH_OK;
 }
 
 int32_t cwbvh_last_status(void) { return g_status; }

@@ -651,6 +651,13 @@ void rtBindBVH2Data(const cwbvh_float4* InBVHTreeNodes,
                     const cwbvh_float4* InTriangleWoopCoordinates,
                     const int* InMappingFromTriangleAddressToIndex);
 void rtTraceBVH2(cwbvh_ray* rayBuffer, cwbvh_hit* rayResultBuffer, int rayCount);
+/* Profiler scoping, the counterpart of the reference's cudaProfilerStart /
+ * cudaProfilerStop (src/TraversalKernelCWBVH.cu:404,423, src/main.cpp:222,236):
+ * resume / pause collection by a profiler that honours roctx (rocprofv3
+ * --selected-regions).  rtTraceCWBVH / rtTraceBVH2 bracket their launch with
+ * them, as the reference does, inside a roctx range named after the call. */
+int32_t cwbvh_profiler_start(void);
+int32_t cwbvh_profiler_stop(void);
 /* Status and kernel time (ms) of the last rt* call on this host thread. */
 int32_t cwbvh_last_status(void);
 float cwbvh_last_trace_ms(void);
