@@ -835,6 +835,45 @@ def bench_c4(args, cw, W, dev) -> dict:
             log(str(row))
             table.append(row)
         srv.close()
+    # scene admission: a new scene of the set's mean size is generated,
+    # built, uploaded and added while host-ray requests arrive at 5k req/s on
+    # the resident scenes; its first request is timed after the add
+    from concurrent.futures import ThreadPoolExecutor as _TP
+
+    admit = {}
+    srv = cw.Server(scenes, cw.default_server_config(memory_budget=8 << 30, max_inflight=3))
+    with _TP(1) as ex:
+        load = ex.submit(srv.run_poisson, 5000.0, 2000, 1000, 1_000_000, pools, hit_pool, 77)
+        entry = {"index": len(specs), "kind": "icospheres", "target_tris": int(tris / len(specs)),
+                 "seed": W.SEEDS["C4"] + 4242}
+        t0 = time.perf_counter()
+        mesh = W.c4_mesh(entry)
+        t1 = time.perf_counter()
+        _, lay_new = cw.build_layouts(mesh, bvh2=False)
+        t2 = time.perf_counter()
+        sc_new = cw.Scene(lay_new)
+        t3 = time.perf_counter()
+        idx = srv.add_scene(sc_new)
+        t4 = time.perf_counter()
+        diag = mesh.diagonal()
+        nrays = cw.rays_ao_surface(mesh, 16384, 8, 1e-4 * diag, 0.0, 0.1 * diag, seed=entry["seed"] + 1)
+        npool = cw.PinnedArray(nrays.shape, np.float32)
+        npool.array[:] = nrays
+        nh = cw.PinnedArray((len(nrays), 4), np.float32)
+        t5 = time.perf_counter()
+        first_ms = srv.wait(srv.submit(idx, npool.ptr, len(nrays), nh.ptr), 60.0)
+        res_load, _ = load.result()
+    ref_new = oracle.trace_cwbvh(lay_new.nodes, lay_new.node_stride, lay_new.woop, nrays, threads=host_threads())
+    admit = {"triangles": mesh.num_triangles, "generate_s": round(t1 - t0, 3), "build_s": round(t2 - t1, 3),
+             "upload_s": round(t3 - t2, 3), "add_ms": round((t4 - t3) * 1e3, 3),
+             "first_request_ms": round(first_ms, 3), "first_request_rays": len(nrays),
+             "first_request_matches_oracle": bool(np.array_equal(nh.array.view(np.uint32), ref_new.view(np.uint32))),
+             "admission_s": round(t4 - t0, 3),
+             "concurrent_load": {"rate_req_s": 5000.0, "p50_ms": round(res_load["p50_ms"], 3),
+                                 "p99_ms": round(res_load["p99_ms"], 3)}}
+    srv.close()
+    log(f"scene admission while serving: {admit}")
+
     # parity: 24 overlapping requests with their own hit buffers, every served
     # hit against the oracle
     rng = np.random.default_rng(W.SEEDS["C4"] + 5)
@@ -882,8 +921,11 @@ def bench_c4(args, cw, W, dev) -> dict:
                           "device": "rays and hits in device memory, used in place (no copies)",
                           "surfels": f"surfels copied in (36 B), {rpt} AO rays generated per surfel in the traversal "
                                      f"kernel, one 4 B hit count per surfel copied out"},
-        "parity": {"rays": checked, "requests": len(reqs), "mismatches": mism, "ok": mism == 0,
-                   "what": "24 overlapping served requests, every hit, closest, bitwise"},
+        "parity": {"rays": checked, "requests": len(reqs), "mismatches": mism,
+                   "ok": mism == 0 and admit["first_request_matches_oracle"],
+                   "what": "24 overlapping served requests, every hit, closest, bitwise; the admitted scene's "
+                           "first request"},
+        "scene_admission": admit,
         "bvh_hbm_gib": round(hbm / 2**30, 2),
         "build_s": round(build_s, 1),
         "device": dev["arch"],

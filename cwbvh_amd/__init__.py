@@ -277,6 +277,7 @@ _SIGS = {
     "cwbvh_server_destroy": (None, [_P]),
     "cwbvh_server_run_poisson": (_I32, [_P, _P, _P, _P, _P, _I64, _P, _P]),
     "cwbvh_server_submit_device": (_I32, [_P, _I32, _P, _I64, _P, C.POINTER(_U64)]),
+    "cwbvh_server_add_scene": (_I32, [_P, _P, C.POINTER(_I32)]),
     "cwbvh_server_submit_surfels": (_I32, [_P, _I32, _P, C.POINTER(_U64)]),
     "cwbvh_server_run_poisson_device": (_I32, [_P, _P, _P, _P, _P, _I64, _P, _P]),
     "cwbvh_server_run_poisson_surfels": (_I32, [_P, _P, _P, _I32, _F, _F, _P, _I64, _P, _P]),
@@ -985,6 +986,13 @@ class Server:
         rid = C.c_uint64()
         check(lib.cwbvh_server_submit(self._s, scene, host_rays_ptr, n, host_hits_ptr, C.byref(rid)))
         return rid.value
+
+    def add_scene(self, scene) -> int:
+        """Admit a resident scene while serving; returns its index."""
+        idx = C.c_int32()
+        check(lib.cwbvh_server_add_scene(self._s, scene.handle, C.byref(idx)))
+        self._scenes.append(scene)
+        return idx.value
 
     def submit_device(self, scene: int, d_rays: int, n: int, d_hits: int) -> int:
         rid = C.c_uint64()

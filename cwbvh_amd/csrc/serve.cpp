@@ -139,6 +139,8 @@ struct Slot {
 }  // namespace
 
 struct cwbvh_server {
+    // reserved to CWBVH_SERVER_MAX_SCENES at creation, so cwbvh_server_add_scene
+    // appends (under mu) without moving the entries the worker reads
     std::vector<const cwbvh_scene*> scenes;
     cwbvh_server_config cfg{};
     char* d_arena = nullptr;
@@ -159,6 +161,10 @@ struct cwbvh_server {
     int inflight = 0;
     double busy_since = 0;
 
+    int32_t num_scenes() {
+        std::lock_guard<std::mutex> g(mu);
+        return (int32_t)scenes.size();
+    }
     int32_t launch(Slot& sl, const std::vector<Request*>& batch);
     void complete(Slot& sl);
     void fail_pending();
@@ -441,7 +447,9 @@ int32_t cwbvh_server_create(const cwbvh_scene* const* scenes, int32_t num_scenes
                          (!(c.flags & CWBVH_TRACE_PRIM_ID) || scenes[i]->prim_idx),
                      "server scenes must be CWBVH scenes");
     if (c.trace.fetch_chunk == 0) c.trace.fetch_chunk = kBatchChunk;  // batches are planned in chunks
+    CW_CHECK_ARG(num_scenes <= CWBVH_SERVER_MAX_SCENES, "too many scenes");
     auto* s = new cwbvh_server();
+    s->scenes.reserve(CWBVH_SERVER_MAX_SCENES);
     s->scenes.assign(scenes, scenes + num_scenes);
     s->cfg = c;
     if (c.serial) s->cfg.max_inflight = 1;
@@ -508,7 +516,7 @@ extern "C" {
 int32_t cwbvh_server_submit(cwbvh_server* s, int32_t scene, const cwbvh_ray* host_rays, int64_t n,
                             cwbvh_hit* host_hits, uint64_t* request_id) {
     CW_CHECK_ARG(s != nullptr && request_id != nullptr, "NULL argument");
-    CW_CHECK_ARG(scene >= 0 && scene < (int32_t)s->scenes.size(), "scene index out of range");
+    CW_CHECK_ARG(scene >= 0 && scene < s->num_scenes(), "scene index out of range");
     CW_CHECK_ARG(n >= 0 && n <= 0x7FFFFFFF && (n == 0 || (host_rays && host_hits)), "bad ray buffer");
     auto* r = new Request();
     r->kind = CWBVH_REQUEST_HOST_RAYS;
@@ -522,7 +530,7 @@ int32_t cwbvh_server_submit(cwbvh_server* s, int32_t scene, const cwbvh_ray* hos
 int32_t cwbvh_server_submit_device(cwbvh_server* s, int32_t scene, const cwbvh_ray* d_rays, int64_t n,
                                    cwbvh_hit* d_hits, uint64_t* request_id) {
     CW_CHECK_ARG(s != nullptr && request_id != nullptr, "NULL argument");
-    CW_CHECK_ARG(scene >= 0 && scene < (int32_t)s->scenes.size(), "scene index out of range");
+    CW_CHECK_ARG(scene >= 0 && scene < s->num_scenes(), "scene index out of range");
     CW_CHECK_ARG(n >= 0 && n <= 0x7FFFFFFF && (n == 0 || (d_rays && d_hits)), "bad ray buffer");
     CW_CHECK_ARG(aligned16(d_rays) && aligned16(d_hits), "ray and hit buffers must be 16-byte aligned");
     auto* r = new Request();
@@ -537,7 +545,7 @@ int32_t cwbvh_server_submit_device(cwbvh_server* s, int32_t scene, const cwbvh_r
 int32_t cwbvh_server_submit_surfels(cwbvh_server* s, int32_t scene, const cwbvh_surfel_request* q,
                                     uint64_t* request_id) {
     CW_CHECK_ARG(s != nullptr && request_id != nullptr && q != nullptr, "NULL argument");
-    CW_CHECK_ARG(scene >= 0 && scene < (int32_t)s->scenes.size(), "scene index out of range");
+    CW_CHECK_ARG(scene >= 0 && scene < s->num_scenes(), "scene index out of range");
     CW_CHECK_ARG(q->num_texels >= 0 && q->rays_per_texel >= 1 &&
                      q->num_texels * (int64_t)q->rays_per_texel <= 0x7FFFFFFF,
                  "bad surfel count");
@@ -549,6 +557,20 @@ int32_t cwbvh_server_submit_surfels(cwbvh_server* s, int32_t scene, const cwbvh_
     r->surf = *q;
     r->n = q->num_texels * (int64_t)q->rays_per_texel;
     return enqueue(s, r, request_id);
+}
+
+int32_t cwbvh_server_add_scene(cwbvh_server* s, const cwbvh_scene* scene, int32_t* index) {
+    CW_CHECK_ARG(s != nullptr && scene != nullptr && index != nullptr, "NULL argument");
+    CW_CHECK_ARG(scene->kind == CWBVH_KIND_CWBVH && (!(s->cfg.flags & CWBVH_TRACE_PRIM_ID) || scene->prim_idx),
+                 "server scenes must be CWBVH scenes");
+    std::lock_guard<std::mutex> g(s->mu);
+    if ((int32_t)s->scenes.size() >= CWBVH_SERVER_MAX_SCENES) {
+        set_error("server holds CWBVH_SERVER_MAX_SCENES scenes already");
+        return CWBVH_ERR_INVALID_ARGUMENT;
+    }
+    *index = (int32_t)s->scenes.size();
+    s->scenes.push_back(scene);
+    return CWBVH_OK;
 }
 
 int32_t cwbvh_server_wait(cwbvh_server* s, uint64_t id, double timeout_s, double* latency_ms) {
@@ -619,7 +641,7 @@ namespace {
 template <class Submit>
 int32_t drive_poisson(cwbvh_server* s, const cwbvh_poisson_spec* spec, const int64_t* pool_sizes, int64_t ring_size,
                       int64_t ray_unit, double* latencies_ms, cwbvh_poisson_result* out, Submit submit) {
-    const int32_t ns = (int32_t)s->scenes.size();
+    const int32_t ns = s->num_scenes();
     std::mt19937_64 rng(spec->seed);
     std::exponential_distribution<double> gap(spec->rate);
     std::uniform_real_distribution<double> u01(0.0, 1.0);
@@ -693,7 +715,7 @@ int32_t cwbvh_server_run_poisson(cwbvh_server* s, const cwbvh_poisson_spec* spec
     CW_CHECK_ARG(s && spec && ray_pools && pool_sizes && hit_pool && out, "NULL argument");
     int32_t rc = check_spec(spec);
     if (rc) return rc;
-    for (size_t k = 0; k < s->scenes.size(); ++k) CW_CHECK_ARG(ray_pools[k] && pool_sizes[k] >= 1, "empty ray pool");
+    for (int32_t k = 0; k < s->num_scenes(); ++k) CW_CHECK_ARG(ray_pools[k] && pool_sizes[k] >= 1, "empty ray pool");
     CW_CHECK_ARG(hit_pool_size >= spec->max_rays, "hit pool smaller than the largest request");
     return drive_poisson(s, spec, pool_sizes, hit_pool_size, 1, latencies_ms, out,
                          [&](int32_t sc, int64_t n, int64_t off, int64_t ring, uint64_t* id) {
@@ -708,7 +730,7 @@ int32_t cwbvh_server_run_poisson_device(cwbvh_server* s, const cwbvh_poisson_spe
     CW_CHECK_ARG(s && spec && d_ray_pools && pool_sizes && d_hit_pool && out, "NULL argument");
     int32_t rc = check_spec(spec);
     if (rc) return rc;
-    for (size_t k = 0; k < s->scenes.size(); ++k) CW_CHECK_ARG(d_ray_pools[k] && pool_sizes[k] >= 1, "empty ray pool");
+    for (int32_t k = 0; k < s->num_scenes(); ++k) CW_CHECK_ARG(d_ray_pools[k] && pool_sizes[k] >= 1, "empty ray pool");
     CW_CHECK_ARG(hit_pool_size >= spec->max_rays, "hit pool smaller than the largest request");
     return drive_poisson(s, spec, pool_sizes, hit_pool_size, 1, latencies_ms, out,
                          [&](int32_t sc, int64_t n, int64_t off, int64_t ring, uint64_t* id) {
@@ -724,7 +746,7 @@ int32_t cwbvh_server_run_poisson_surfels(cwbvh_server* s, const cwbvh_poisson_sp
     int32_t rc = check_spec(spec);
     if (rc) return rc;
     CW_CHECK_ARG(rays_per_texel >= 1 && spec->max_rays >= rays_per_texel, "bad rays per texel");
-    const size_t ns = s->scenes.size();
+    const size_t ns = (size_t)s->num_scenes();
     std::vector<int64_t> sizes(ns);
     for (size_t k = 0; k < ns; ++k) {
         CW_CHECK_ARG(pools[k].positions && pools[k].normals && pools[k].radius && pools[k].num_texels >= 1,

@@ -489,6 +489,8 @@ typedef struct cwbvh_server_config {
 
 int32_t cwbvh_server_config_default(cwbvh_server_config* cfg);
 
+/* Most scenes one server holds (at creation and added later). */
+#define CWBVH_SERVER_MAX_SCENES 4096
 /* Resident scenes (CWBVH, not owned; they must outlive the server). */
 int32_t cwbvh_server_create(const cwbvh_scene* const* scenes, int32_t num_scenes,
                             const cwbvh_server_config* cfg, cwbvh_server** out);
@@ -498,6 +500,12 @@ int32_t cwbvh_server_create(const cwbvh_scene* const* scenes, int32_t num_scenes
  * than the memory budget fail with CWBVH_ERR_BUDGET. */
 int32_t cwbvh_server_submit(cwbvh_server* s, int32_t scene, const cwbvh_ray* host_rays, int64_t n,
                             cwbvh_hit* host_hits, uint64_t* request_id);
+/* Admit another resident scene while the server runs (CWBVH, not owned; it
+ * must outlive the server).  *index receives its scene index, the next free
+ * one; requests may name it as soon as the call returns.  The scene's build
+ * and upload happen before this call, so they bound how fast a new scene
+ * becomes servable, not the latency of the requests already admitted. */
+int32_t cwbvh_server_add_scene(cwbvh_server* s, const cwbvh_scene* scene, int32_t* index);
 /* Request kinds (cwbvh_server_submit, _submit_device, _submit_surfels). */
 enum {
     CWBVH_REQUEST_HOST_RAYS = 0,    /* rays copied in, hits copied out (48 B per ray over PCIe) */

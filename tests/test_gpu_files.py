@@ -18,7 +18,9 @@ def bits(h):
     return np.ascontiguousarray(h, np.float32).view(np.uint32)
 
 
-def test_layout_cache_scene_traces_identically(cw, tmp_path):
+def test_layout_cache_scene_traces_identically(cw, orc, tmp_path):
+    """A scene reloaded from the layout cache returns the oracle's hits (on
+    the reloaded arrays, which equal the built ones) and the built scene's."""
     mesh = cw.Mesh.icosphere_grid(3, 3, 3, 4, 4.0, 0.5, 1.5, ground_margin=0.5, seed=8)
     diag = mesh.diagonal()
     rays = cw.rays_ao_surface(mesh, 20000, 8, 1e-4 * diag, 0.0, 0.1 * diag, seed=9)
@@ -26,10 +28,18 @@ def test_layout_cache_scene_traces_identically(cw, tmp_path):
         say(f"layout cache kind={lay.kind}")
         p = tmp_path / f"k{lay.kind}.lay"
         lay.save(p)
+        re = cw.Layout.load(p)
+        assert np.array_equal(bits(re.nodes), bits(lay.nodes)) and np.array_equal(bits(re.woop), bits(lay.woop))
         a = cw.Scene(lay)
-        b = cw.Scene(cw.Layout.load(p))
+        b = cw.Scene(re)
         for fl in (cw.TRACE_CLOSEST, cw.TRACE_ANY):
-            assert np.array_equal(bits(a.trace_numpy(rays, fl)), bits(b.trace_numpy(rays, fl)))
+            if re.kind == cw.KIND_CWBVH:
+                ref = orc.trace_cwbvh(re.nodes, re.node_stride, re.woop, rays, any_hit=bool(fl))
+            else:
+                ref = orc.trace_bvh2(re.nodes, re.woop, rays, any_hit=bool(fl))
+            got = b.trace_numpy(rays, fl)
+            assert np.array_equal(bits(got), bits(ref)), "reloaded scene differs from the oracle"
+            assert np.array_equal(bits(a.trace_numpy(rays, fl)), bits(got))
 
 
 def test_reference_file_workload(cw, orc, tmp_path):

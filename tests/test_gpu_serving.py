@@ -227,3 +227,37 @@ def test_poisson_device_and_surfel_harnesses(cw, serving_set, surfel_pools):
     assert res["rays"] > 0 and 0 < res["p50_ms"] <= res["p99_ms"] and np.all(lat > 0)
     assert srv.stats()["completed"] == 120
     srv.close()
+
+
+def test_scene_admitted_while_serving(cw, serving_set):
+    """A scene built and admitted while requests on the resident scenes are in
+    flight serves its own requests with the oracle's hits; the earlier
+    requests are unaffected."""
+    from cwbvh_amd import workloads as W
+
+    scenes, pools, lays = serving_set
+    srv = cw.Server(scenes[:2], cw.default_server_config(max_inflight=3, memory_budget=64 << 20))
+    early = []
+    for k in range(10):
+        n = 20000
+        h = cw.PinnedArray((n, 4), np.float32)
+        early.append((srv.submit(k % 2, pools[k % 2].ptr, n, h.ptr), k % 2, n, h))
+    mesh = W.icospheres_of(25000, seed=909)
+    _, lay = cw.build_layouts(mesh, bvh2=False)
+    new = cw.Scene(lay)
+    idx = srv.add_scene(new)
+    assert idx == 2
+    diag = mesh.diagonal()
+    rays = cw.rays_ao_surface(mesh, 5000, 8, 1e-4 * diag, 0.0, 0.1 * diag, seed=910)
+    pool = cw.PinnedArray(rays.shape, np.float32)
+    pool.array[:] = rays
+    h = cw.PinnedArray((len(rays), 4), np.float32)
+    rid = srv.submit(idx, pool.ptr, len(rays), h.ptr)
+    srv.wait(rid, 60.0)
+    assert np.array_equal(bits(h.array), bits(oracle_hits(lay, rays)))
+    for rid, s, n, hh in early:
+        srv.wait(rid, 60.0)
+        assert np.array_equal(bits(hh.array), bits(oracle_hits(lays[s], pools[s].array[:n])))
+    with pytest.raises(cw.CwbvhError):
+        srv.submit(3, pool.ptr, 10, h.ptr)   # no scene 3
+    srv.close()
