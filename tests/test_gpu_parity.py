@@ -100,6 +100,13 @@ CWBVH_VARIANTS = [
     dict(loop_mode=2, postpone_pct=100, block_size=64, fetch_chunk=3, prefetch=1),
     dict(loop_mode=1, lds_depth=2, persistent=0),
     dict(loop_mode=2, postpone_pct=40, block_size=256, refill_idle=0, refill_budget=1),
+    dict(loop_mode=3),
+    dict(loop_mode=3, postpone_pct=2),
+    dict(loop_mode=3, postpone_pct=100, block_size=256, lds_depth=4),
+    dict(loop_mode=3, persistent=0, block_size=64),
+    dict(loop_mode=3, prefetch=1, fetch_chunk=7, lds_depth=2),
+    dict(loop_mode=3, refill_idle=0, refill_budget=1, work_slices=3),
+    dict(loop_mode=3, block_size=64, lds_depth=64, refill_idle=64, refill_budget=1 << 20),
 ]
 
 
