@@ -151,6 +151,10 @@ __device__ __forceinline__ unsigned child_hits4(unsigned meta4, unsigned lox, un
 //    earliest accepted test in the queue, which is the first in per-ray
 //    order.  Lanes keep visiting nodes with the interval they have until
 //    their tests run, which can only add visits.
+//    Switched off: no launch instantiates this schedule and a configuration
+//    that asks for it is refused (validate_config).  A GPU run that included
+//    its first launches faulted the device, together with another untested
+//    path, and the cause has not been shown.
 constexpr int kLoopGroup = 0;
 constexpr int kLoopInterleave = 1;
 constexpr int kLoopSpeculate = 2;
@@ -988,7 +992,6 @@ int32_t dispatch_loop(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t
             }
             return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopInterleave>(p, cfg, st);
         case kLoopSpeculate: return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopSpeculate>(p, cfg, st);
-        case kLoopQueue: return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopQueue>(p, cfg, st);
         default: return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopGroup>(p, cfg, st);
     }
 }
@@ -1017,7 +1020,6 @@ int32_t dispatch_surfels(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStrea
             }
             return launch_cwbvh_t<ANY, STATS, false, false, kLoopInterleave, kRaysFromSurfels>(p, cfg, st);
         case kLoopSpeculate: return launch_cwbvh_t<ANY, STATS, false, false, kLoopSpeculate, kRaysFromSurfels>(p, cfg, st);
-        case kLoopQueue: return launch_cwbvh_t<ANY, STATS, false, false, kLoopQueue, kRaysFromSurfels>(p, cfg, st);
         default: return launch_cwbvh_t<ANY, STATS, false, false, kLoopGroup, kRaysFromSurfels>(p, cfg, st);
     }
 }
@@ -1068,6 +1070,10 @@ int32_t validate_config(int32_t kind, const cwbvh_trace_config& c) {
             c.loop_mode < kLoopGroup || c.loop_mode > kLoopQueue ||
             !(c.occupancy == kOccupancyAuto || c.occupancy == kOccupancyFree || c.occupancy == kOccupancy7)) {
             set_error("invalid CWBVH trace configuration");
+            return CWBVH_ERR_INVALID_ARGUMENT;
+        }
+        if (c.loop_mode == kLoopQueue) {
+            set_error("loop_mode 3 (triangle queue) is switched off: use 0, 1 or 2");
             return CWBVH_ERR_INVALID_ARGUMENT;
         }
         if (c.postpone_pct > 0 && c.loop_mode == kLoopInterleave) {

@@ -351,7 +351,8 @@ typedef struct cwbvh_trace_config {
     int32_t loop_mode;       /* CWBVH inner-loop schedule: 0 = node visit + its whole triangle group
                                 per iteration (reference order), 1 = node visit or one triangle test
                                 per iteration, 2 = as 1, and lanes holding triangles also visit a
-                                node, parking its triangle group on the stack */
+                                node, parking its triangle group on the stack; 3 (per-wave triangle
+                                queue) is switched off and refused with CWBVH_ERR_INVALID_ARGUMENT */
     int32_t occupancy;       /* register budget of the single-scene loop_mode 1 kernels (no prefetch,
                                 no stats; rays from a buffer or from surfels): 0 = automatic
                                 (7 waves per SIMD for launches of 2^22 rays or more), 1 = unconstrained,

@@ -123,7 +123,13 @@ def assert_any_hit_valid(orc, lay, rays, g, ref, what):
         assert ok and np.float32(t) == g[i, 0] and np.float32(u) == g[i, 2] and np.float32(v) == g[i, 3], (what, i)
 
 
-@pytest.mark.parametrize("variant", range(len(CWBVH_VARIANTS)))
+QUEUE_OFF = ("loop_mode 3 is switched off: a GPU run that included its first launches faulted the device "
+             "together with another untested path, and no cause has been shown")
+
+
+@pytest.mark.parametrize("variant", [pytest.param(i, marks=pytest.mark.skip(reason=QUEUE_OFF))
+                                     if CWBVH_VARIANTS[i].get("loop_mode") == 3 else i
+                                     for i in range(len(CWBVH_VARIANTS))])
 def test_cwbvh_config_variants_bit_identical(cw, orc, c0, c0_ref, variant):
     """Every schedule is bit-identical to the oracle for closest hit; for
     any-hit too, except speculation (loop_mode 2), whose contract is
