@@ -273,7 +273,7 @@ def bench_single(args, cw, W, dev, name: str) -> dict:
     def run(sc, fl, cfg, dr=d_rays, dh=d_hits, cnt=n):
         return lambda: sc.trace(dr.ptr, dh.ptr, cnt, fl, cfg)
 
-    # the timed launches are a profiler region (rocprofv3 --selected-regions
+    # the timed launches are a profiler region (rocprofv3 --selected-regions --marker-trace
     # collects these and nothing of the set-up, counting or parity launches)
     log("timing CWBVH closest + any")
     cw.profiler_start()
@@ -958,7 +958,7 @@ def profile_launches(args, cw, W) -> dict:
     src, dst = cw.DeviceBuffer(copy_bytes), cw.DeviceBuffer(copy_bytes)
     src.zero()
     cw.synchronize()
-    # everything below is one profiler region (rocprofv3 --selected-regions)
+    # everything below is one profiler region (rocprofv3 --selected-regions --marker-trace)
     cw.profiler_start()
     cw.check(cw.lib.cwbvh_copy_u4(dst.ptr, src.ptr, copy_bytes // 16, None))
     probe_blocks = 8192

@@ -866,7 +866,7 @@ def morton_order(d_rays: int, n: int, lo, hi, d_order: int, stream: Stream | Non
 
 
 def profiler_start() -> None:
-    """Resume profiler collection (rocprofv3 --selected-regions)."""
+    """Resume profiler collection (rocprofv3 --selected-regions --marker-trace)."""
     check(lib.cwbvh_profiler_start())
 
 

@@ -66,7 +66,7 @@ void trace(cwbvh_scene* scene, cwbvh_ray* rays, cwbvh_hit* hits, int count, cons
     }
     // the reference brackets each trace with cudaProfilerStart / Stop
     // (src/TraversalKernelCWBVH.cu:404,423): the roctx counterparts scope
-    // rocprofv3 --selected-regions to the launch
+    // rocprofv3 --selected-regions --marker-trace to the launch
     roctxProfilerResume(0);
     roctxRangePush(label);
     hipEventRecord(a, nullptr);

@@ -430,7 +430,7 @@ int32_t cwbvh_scene_bounds(const cwbvh_scene* scene, float* lo3, float* hi3);
 /* Morton ray order: d_order[i] = index of the i-th ray when rays are sorted by
  * the 30-bit Morton code of their origin quantized inside [lo, hi].
  * d_order holds n int32.  Stream-ordered (the call does not wait).  The
- * keys and sort temporaries (about 12 B per ray) live in a workspace kept per
+ * keys and sort temporaries (about 16 B per ray) live in a workspace kept per
  * device and stream and grown on demand, so repeated calls do not allocate;
  * cwbvh_stream_destroy releases a stream's workspace. */
 int32_t cwbvh_morton_order(const cwbvh_ray* d_rays, int64_t n, const float* lo3, const float* hi3,
@@ -663,8 +663,9 @@ void rtTraceBVH2(cwbvh_ray* rayBuffer, cwbvh_hit* rayResultBuffer, int rayCount)
 /* Profiler scoping, the counterpart of the reference's cudaProfilerStart /
  * cudaProfilerStop (src/TraversalKernelCWBVH.cu:404,423, src/main.cpp:222,236):
  * resume / pause collection by a profiler that honours roctx (rocprofv3
- * --selected-regions).  rtTraceCWBVH / rtTraceBVH2 bracket their launch with
- * them, as the reference does, inside a roctx range named after the call. */
+ * --selected-regions --marker-trace).  rtTraceCWBVH / rtTraceBVH2 bracket
+ * their launch with them, as the reference does, inside a roctx range named
+ * after the call. */
 int32_t cwbvh_profiler_start(void);
 int32_t cwbvh_profiler_stop(void);
 /* Status and kernel time (ms) of the last rt* call on this host thread. */

@@ -193,8 +193,9 @@ def test_morton_order_reuses_its_workspace(cw):
         assert np.array_equal(order, np.argsort(_morton_codes(rays, lo, hi), kind="stable"))
 
     st = cw.Stream()
-    # sizes around the sort's 4096-key tiles and 64-key waves
-    for n in (50_000, 1_000, 300_000, 7, 1, 64, 65, 4096, 4097, 8191, (1 << 20) + 5):
+    # sizes around the sort's 4096-key tiles and 64-key waves, and one of
+    # 4,100 tiles (65 chunks of the column scan)
+    for n in (50_000, 1_000, 300_000, 7, 1, 64, 65, 4096, 4097, 8191, (1 << 20) + 5, (1 << 24) + 12_345):
         check(n, st)
     del st
     gc.collect()
