@@ -466,8 +466,8 @@ int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_confi
     // VALU-bound and gains from the interleaved schedule (C1: 1.36x).  A
     // larger one is bound by HBM traffic.  Sorting a launch's rays by origin
     // first lets neighbouring waves share the nodes they fetch (C3, 1.7 GB,
-    // 67 M shuffled rays: the 2.5 ms sort and the Morton-ordered trace
-    // together are 1.19x the trace in the given order), and sorted rays run
+    // 67 M shuffled rays: the 2.0 ms sort and the sorted trace together are
+    // 1.23x the trace in the given order), and sorted rays run
     // best with the speculative schedule (C3: 1.05x over grouped) and node
     // prefetch, whose loads of the next node overlap the triangle tests
     // (C3, 16.8 M rays: 1.07x; deeper LDS stacks and other refill budgets

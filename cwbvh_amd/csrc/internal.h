@@ -61,13 +61,17 @@ void release_stream_scratch(hipStream_t stream);
 // The lock that guards every stream's workspace: hold it from a workspace
 // lookup until the kernels that use the workspace are queued.
 std::unique_lock<std::mutex> lock_stream_scratch();
-// With that lock held: origin Morton order of n rays (n >= 1) into the
-// stream's workspace; *order points into it until the stream's next use.
+// Ray order for coherent traversal: rays sorted stably by a kRayKeyBits-bit
+// key, their direction octant above the Morton code of their origin
+// quantized to 7 bits per axis inside the scene bounds.
+constexpr int kRayKeyBits = 24;
+// With that lock held: the ray order of n rays (n >= 1) into the stream's
+// workspace; *order points into it until the stream's next use.
 int32_t morton_order_scratch(const void* d_rays, int64_t n, const float lo[3], const float hi[3],
                              hipStream_t stream, const int32_t** order);
-// Origin Morton order of n rays into `order`, with caller buffers: keys k0
-// and k1 and values va and vb (n entries each) and the sort's counts ws
-// (radix_sort_workspace(n, 30) bytes).  Stream-ordered.
+// The ray order of n rays into `order`, with caller buffers: keys k0 and k1
+// and values va and vb (n entries each) and the sort's counts ws
+// (radix_sort_workspace(n, kRayKeyBits) bytes).  Stream-ordered.
 int32_t morton_order_into(const void* d_rays, int64_t n, const float lo3[3], const float hi3[3], uint32_t* k0,
                           uint32_t* k1, int32_t* va, int32_t* vb, int32_t* order, void* ws, size_t ws_bytes,
                           hipStream_t st);

@@ -358,9 +358,9 @@ typedef struct cwbvh_trace_config {
                                 (7 waves per SIMD for launches of 2^22 rays or more), 1 = unconstrained,
                                 7 = 7 waves per SIMD (72 VGPRs) */
     int32_t ray_order;       /* 0 = trace rays in the given order; 1 = a launch of 2^20 rays or more
-                                first sorts them by origin Morton code (as cwbvh_morton_order, in the
-                                stream's workspace) and traces them in that order, in pieces of at
-                                most 2^26 rays.  Hits still go to each ray's own slot, so results do
+                                first sorts them by direction octant and origin Morton code (as
+                                cwbvh_morton_order, in the stream's workspace) and traces them in
+                                that order, in pieces of at most 2^26 rays.  Hits still go to each ray's own slot, so results do
                                 not change.  Applies to cwbvh_trace; ignored when an order is given,
                                 by surfel traces, and by batches and the server (batches have their
                                 own Morton pre-pass, the `morton` argument) */
@@ -427,8 +427,11 @@ int32_t cwbvh_loop_profile(void* stream, uint64_t* out6);
  * at creation (the unit cube when the root holds no child box). */
 int32_t cwbvh_scene_bounds(const cwbvh_scene* scene, float* lo3, float* hi3);
 
-/* Morton ray order: d_order[i] = index of the i-th ray when rays are sorted by
- * the 30-bit Morton code of their origin quantized inside [lo, hi].
+/* Ray order for coherent traversal: d_order[i] = index of the i-th ray when
+ * rays are sorted stably by a 24-bit key, the octant of their direction (3
+ * bits, sign of x highest) above the 21-bit Morton code of their origin
+ * quantized to 7 bits per axis inside [lo, hi] (q = trunc(clamp((o - lo) *
+ * (128 / (hi - lo)), 0, 127)) in float; x bit highest in each triple).
  * d_order holds n int32.  Stream-ordered (the call does not wait).  The
  * keys and sort temporaries (about 16 B per ray) live in a workspace kept per
  * device and stream and grown on demand, so repeated calls do not allocate;
@@ -455,8 +458,8 @@ typedef struct cwbvh_batch cwbvh_batch;
 
 /* Plan one persistent launch that traverses every item: a device job table
  * (one 64-byte record per item) and private fetch counters.  morton != 0 adds
- * a per-launch pre-pass that orders every item's rays by origin Morton code
- * (key = item index : Morton code, one radix sort over all rays). */
+ * a per-launch pre-pass that orders every item's rays as cwbvh_morton_order
+ * does (key = item index : ray key, one radix sort over all rays). */
 int32_t cwbvh_batch_create(const cwbvh_batch_item* items, int32_t count, uint32_t flags,
                            const cwbvh_trace_config* cfg, int32_t morton, cwbvh_batch** out);
 /* Morton pre-pass (if enabled) + the persistent traversal, on `stream`. */

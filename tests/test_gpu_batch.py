@@ -140,39 +140,34 @@ def test_morton_order_single_scene(cw, orc, batch_set):
     cw.morton_order(dr.ptr, n, lo, hi, do.ptr)
     order = do.download((n,), np.int32)
     assert np.array_equal(np.sort(order), np.arange(n))
-    # sorted by the Morton code of the quantized origins
-    q = np.clip(((rays[order, :3] - lo) * (1024.0 / np.maximum(hi - lo, 1e-30))), 0, 1023).astype(np.int64)
-
-    def spread(v):
-        out = np.zeros_like(v)
-        for b in range(10):
-            out |= ((v >> b) & 1) << (3 * b)
-        return out
-
-    code = (spread(q[:, 0]) << 2) | (spread(q[:, 1]) << 1) | spread(q[:, 2])
-    assert np.all(np.diff(code) >= 0)
+    # sorted by the ray key (direction octant, then origin Morton code)
+    assert np.all(np.diff(_ray_keys(rays[order], lo, hi)) >= 0)
     g = sc.trace_numpy(rays, 0, order=order)
     assert np.array_equal(bits(g), bits(refs[7][False]))
 
 
-def _morton_codes(rays, lo, hi):
-    """Keys of the Morton pre-pass in float32: scale = 1024 / (hi - lo),
-    q = trunc(clamp((o - lo) * scale, 0, 1023)) per axis, x bits highest."""
+def _ray_keys(rays, lo, hi):
+    """Keys of the ray-order pre-pass, in float32: scale = 128 / (hi - lo),
+    q = trunc(clamp((o - lo) * scale, 0, 127)) per axis; key = direction
+    octant (x sign highest) << 21 | Morton code of q (x bit highest)."""
     lo, hi = np.float32(lo), np.float32(hi)
-    scale = np.float32(1024.0) / (hi - lo)
-    f = np.minimum(np.maximum((rays[:, :3] - lo) * scale, np.float32(0)), np.float32(1023))
+    scale = np.float32(128.0) / (hi - lo)
+    f = np.minimum(np.maximum((rays[:, :3] - lo) * scale, np.float32(0)), np.float32(127))
     q = f.astype(np.uint32).astype(np.int64)
     code = np.zeros(len(rays), np.int64)
-    for b in range(10):
+    for b in range(7):
         for axis, shift in ((0, 2), (1, 1), (2, 0)):
             code |= ((q[:, axis] >> b) & 1) << (3 * b + shift)
-    return code
+    d = rays[:, 4:7]
+    octant = ((d[:, 0] < 0).astype(np.int64) << 2) | ((d[:, 1] < 0).astype(np.int64) << 1) | (d[:, 2] < 0)
+    return (octant << 21) | code
 
 
 def test_morton_order_reuses_its_workspace(cw):
     """Calls of shrinking and growing size on one stream reuse and grow the
     kept workspace; each order is the stable sort of the float32 keys (ties
-    keep ray order), origins outside [lo, hi] clamp; destroying the stream
+    keep ray order; directions in every octant), origins outside [lo, hi]
+    clamp; destroying the stream
     releases its workspace and later calls on the null stream still work."""
     rng = np.random.default_rng(21)
     lo = np.array([-1.0, -2.0, -3.0], np.float32)
@@ -183,14 +178,16 @@ def test_morton_order_reuses_its_workspace(cw):
         rays = np.zeros((n, 8), np.float32)
         rays[:, :3] = rng.uniform(lo - 0.1, hi + 0.1, size=(n, 3)).astype(np.float32)
         rays[: n // 4, :3] = np.round(rays[: n // 4, :3], 1)   # repeated keys
-        rays[:, 4], rays[:, 7] = 1.0, np.inf
+        rays[:, 4:7] = rng.normal(size=(n, 3)).astype(np.float32)
+        rays[: n // 8, 4:7] = np.abs(rays[: n // 8, 4:7])      # one octant
+        rays[:, 7] = np.inf
         dr = cw.DeviceBuffer.from_numpy(rays)
         do = cw.DeviceBuffer(4 * n)
         cw.morton_order(dr.ptr, n, lo, hi, do.ptr, stream)
         if stream is not None:
             stream.synchronize()
         order = do.download((n,), np.int32)
-        assert np.array_equal(order, np.argsort(_morton_codes(rays, lo, hi), kind="stable"))
+        assert np.array_equal(order, np.argsort(_ray_keys(rays, lo, hi), kind="stable"))
 
     st = cw.Stream()
     # sizes around the sort's 4096-key tiles and 64-key waves, and one of
