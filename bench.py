@@ -619,7 +619,7 @@ def bench_c3(args, cw, W, dev) -> dict:
         "cwbvh_vs_bvh2": {"default": round(res["bvh2_default"]["ms"] / res["cwbvh_default"]["ms"], 3),
                           "shuffled": round(res["bvh2_shuffled"]["ms"] / res["cwbvh_shuffled"]["ms"], 3),
                           "morton": round(res["bvh2_morton"]["ms"] / res["cwbvh_morton"]["ms"], 3)},
-        "roofline": roofline("cwbvh_trace<closest> (Morton pre-pass + speculative schedule, 128 B nodes)", n, ms_c,
+        "roofline": roofline("cwbvh_trace<closest> (ray-order pre-pass + interleaved schedule with prefetch, 128 B nodes)", n, ms_c,
                              s_cw.device_bytes(), bytes_c,
                              {"nodes": int(c[0]) / n, "triangles": int(c[1]) / n, "bytes": bytes_c / n},
                              traffic=rec.get("hbm_bytes_per_launch") if rec else None),

@@ -464,21 +464,22 @@ int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_confi
     if (rc) return rc;
     // measured on MI355X: a tree that fits the 256 MB Infinity Cache is
     // VALU-bound and gains from the interleaved schedule (C1: 1.36x).  A
-    // larger one is bound by HBM traffic.  Sorting a launch's rays by origin
-    // first lets neighbouring waves share the nodes they fetch (C3, 1.7 GB,
-    // 67 M shuffled rays: the 2.0 ms sort and the sorted trace together are
-    // 1.23x the trace in the given order), and sorted rays run
-    // best with the speculative schedule (C3: 1.05x over grouped) and node
-    // prefetch, whose loads of the next node overlap the triangle tests
-    // (C3, 16.8 M rays: 1.07x; deeper LDS stacks and other refill budgets
-    // add nothing on top).  A tree without bounds (an empty root) cannot
+    // larger one is bound by HBM traffic.  Sorting a launch's rays by
+    // direction octant and origin first lets neighbouring waves share the
+    // nodes they fetch and their child order (C3, 1.7 GB, 67 M shuffled rays:
+    // the 2.0 ms sort and the sorted trace together are 1.23x the trace in the
+    // given order), and sorted rays run best with the interleaved schedule and
+    // node prefetch, whose loads of the next node overlap the triangle tests
+    // (C3 closest hit on sorted rays: 49.6 ms, against 50.7 speculative with
+    // prefetch, 56.0 grouped, 58.5 interleaved without prefetch; block 256
+    // and deeper LDS stacks add nothing).  A tree without bounds (an empty root) cannot
     // quantize origins; its unsorted rays keep the grouped schedule, which
     // tests a leaf's triangles right after fetching the node (C3 shuffled:
     // 1.10x over interleaved).
     if (scene->bytes > kCacheResidentBytes) {
         if (scene->bounds_known) c->ray_order = 1;
         if (scene->kind == CWBVH_KIND_CWBVH) {
-            c->loop_mode = scene->bounds_known ? 2 : 0;
+            c->loop_mode = scene->bounds_known ? 1 : 0;
             c->prefetch = scene->bounds_known ? 1 : 0;
         }
     }

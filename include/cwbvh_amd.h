@@ -372,7 +372,7 @@ int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
  * except for scenes larger than the 256 MB Infinity Cache, whose traversal is
  * bound by HBM traffic.  Such a scene gets the Morton ray pre-pass
  * (ray_order 1, in bounds read from its root node) and, for CWBVH, the
- * speculative schedule with node prefetch (loop_mode 2, prefetch 1); one
+ * interleaved schedule with node prefetch (loop_mode 1, prefetch 1); one
  * whose root holds no child box gets the grouped schedule (loop_mode 0).
  * When the pre-pass cannot allocate its workspace, the launch traces the rays
  * in the given order. */

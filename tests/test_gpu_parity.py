@@ -326,7 +326,7 @@ def test_large_scene_uploads_128b_nodes(cw):
     keep = cw.Scene(cwb, stride_mode=cw.SCENE_STRIDE_KEEP)
     assert auto.node_stride == 8 and keep.node_stride == 5
     d = auto.default_config()
-    assert d.loop_mode == 2 and d.prefetch == 1 and d.ray_order == 1
+    assert d.loop_mode == 1 and d.prefetch == 1 and d.ray_order == 1
     as_given = keep.default_config()
     as_given.ray_order = 0
     diag = mesh.diagonal()
@@ -341,7 +341,7 @@ def test_large_scene_uploads_128b_nodes(cw):
     bufs = [cw.DeviceBuffer.from_numpy(a) for a in (cwb.nodes, cwb.woop, cwb.prim_idx)]
     wrapped = cw.Scene.wrap(cw.KIND_CWBVH, 5, bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, st["stack_need"], keep=bufs)
     d = wrapped.default_config()
-    assert d.ray_order == 1 and d.loop_mode == 2 and d.prefetch == 1
+    assert d.ray_order == 1 and d.loop_mode == 1 and d.prefetch == 1
     n = len(rays)
     d_rays = cw.DeviceBuffer.from_numpy(rays)
     d_hits = cw.DeviceBuffer(16 * n)
