@@ -137,34 +137,14 @@ __device__ __forceinline__ unsigned child_hits4(unsigned meta4, unsigned lox, un
 //    A lane speculates only while its stack holds fewer than spec_limit
 //    entries (capacity minus the tree's stack need), so the parked groups
 //    never push the stack past its capacity.
-//  kLoopQueue: a lane visits a node whenever it has one; the triangle group
-//    a visit yields goes, highest triangle first, into a queue of (lane,
-//    triangle) tests the wave keeps in LDS.  When the queue holds enough tests
-//    (postpone_pct % of 64, default 75%), when a group does not fit, when
-//    several lanes have nothing left but queued tests, or when no lane has a
-//    node to visit, the wave's active lanes take the queued tests one each
-//    (the ray's origin, direction and interval come from its lane by
-//    permute) and the results go back to their lanes through LDS minima.
-//    Each lane's tests keep their per-ray order; a closest hit is the
-//    smallest (t, address) among the accepted tests, which is what testing
-//    them one by one gives under the tie rule, and an any hit is the
-//    earliest accepted test in the queue, which is the first in per-ray
-//    order.  Lanes keep visiting nodes with the interval they have until
-//    their tests run, which can only add visits.
-//    Switched off: no launch instantiates this schedule and a configuration
-//    that asks for it is refused (validate_config).  A GPU run that included
-//    its first launches faulted the device, together with another untested
-//    path, and the cause has not been shown.
+//  loop_mode 3 (a per-wave queue of triangle tests run 64 at a time) is
+//    refused by validate_config: its first GPU run faulted the device
+//    together with another untested path, and the cause was not shown, so the
+//    schedule was taken out.
 constexpr int kLoopGroup = 0;
 constexpr int kLoopInterleave = 1;
 constexpr int kLoopSpeculate = 2;
 constexpr int kLoopQueue = 3;
-
-// LDS of the triangle queue, per wave, after the traversal stacks: 64 tests
-// (owner lane, Woop address), 64 result keys, 64 results (t, u, v) and the
-// wave's test count.
-constexpr int kQueueTests = 64;
-constexpr size_t kQueueBytesPerWave = kQueueTests * 8 + kQueueTests * 8 + kQueueTests * 16 + 16;
 
 // Register budget (cwbvh_trace_config.occupancy).  The plain interleaved
 // kernel (one scene, no counters, no prefetch) takes 75 VGPRs unconstrained
@@ -195,24 +175,6 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
     unsigned long long spill[kStackCapacity];
     const StackIndex sx((int)threadIdx.x, P.lds_depth);
     const int lane = lane_id();
-    // kLoopQueue: this wave's queue region behind the block's stacks
-    CWD_LDS unsigned long long* q_test = nullptr;   // Woop address << 32 | owner lane
-    CWD_LDS unsigned long long* q_key = nullptr;
-    CWD_LDS float* q_res = nullptr;                  // t, u, v (and a pad) per owner
-    CWD_LDS unsigned* q_count = nullptr;
-    if constexpr (LOOP == kLoopQueue) {
-        CWD_LDS char* qb = reinterpret_cast<CWD_LDS char*>(lds_stack + (size_t)blockDim.x * P.lds_depth) +
-                           (threadIdx.x >> 6) * kQueueBytesPerWave;
-        q_test = reinterpret_cast<CWD_LDS unsigned long long*>(qb);
-        q_key = reinterpret_cast<CWD_LDS unsigned long long*>(qb + kQueueTests * 8);
-        q_res = reinterpret_cast<CWD_LDS float*>(qb + kQueueTests * 16);
-        q_count = reinterpret_cast<CWD_LDS unsigned*>(qb + kQueueTests * 32);
-        if (lane == 0) *q_count = 0u;
-    }
-    int mine = 0;   // kLoopQueue: this lane's tests in the queue
-    const unsigned flush_at = LOOP == kLoopQueue
-                                  ? (P.postpone_pct > 0 ? max(1u, (unsigned)(P.postpone_pct * kQueueTests) / 100u) : 48u)
-                                  : 0u;
 
     // per-lane ray state
     int out_idx = -1;
@@ -536,111 +498,6 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                         finished = true;
                     }
                 }
-            } else if constexpr (LOOP == kLoopQueue) {
-                // node step: every lane with a node to visit and no group
-                // waiting for room in the queue
-                const bool node_step = tg.y == 0u && ng.y > 0x00FFFFFFu;
-                prof(node_step, pr_nd, pr_ndu);
-                if (node_step) tg = visit_node();
-                // enqueue: lanes whose whole group fits append its tests,
-                // highest triangle first (lanes in order, so the ones that fit
-                // are a prefix of the lanes holding groups)
-                unsigned qn = *q_count;
-                const unsigned c = (unsigned)__popc(tg.y);
-                unsigned pre = 0, tot = 0;
-#pragma unroll
-                for (int b = 0; b < 5; ++b) {
-                    const unsigned long long bb = __ballot((c >> b) & 1u);
-                    pre += (unsigned)__popcll(bb & lanemask_lt()) << b;
-                }
-                const bool fits = c != 0u && qn + pre + c <= (unsigned)kQueueTests;
-#pragma unroll
-                for (int b = 0; b < 5; ++b) tot += (unsigned)__popcll(__ballot(fits && ((c >> b) & 1u))) << b;
-                if (fits) {
-                    unsigned slot = qn + pre, m = tg.y;
-                    const unsigned tri0 = tg.x * 3u;
-                    while (m != 0u) {
-                        const int ti = 31 - __clz((int)m);
-                        m &= ~(1u << ti);
-                        q_test[slot++] = ((unsigned long long)(tri0 + (unsigned)ti * 3u) << 32) | (unsigned)lane;
-                    }
-                    mine += (int)c;
-                    tg.y = 0u;
-                }
-                qn += tot;
-                // out of inner children and triangles: the next node group
-                if (ng.y <= 0x00FFFFFFu && tg.y == 0u && sp > 0) {
-                    --sp;
-                    ng = unpack2(stack_get(lds_stack, spill, sx, sp));
-                }
-                const bool blocked = tg.y != 0u;
-                const bool waiting = ng.y <= 0x00FFFFFFu && tg.y == 0u && mine > 0;
-                const bool flush = qn > 0u && (qn >= flush_at || __any(blocked) || __popcll(__ballot(waiting)) >= 8 ||
-                                               !__any(ng.y > 0x00FFFFFFu));
-                if (flush) {
-                    // every active lane takes queued tests in turn; owners
-                    // are active (a lane with queued tests never leaves)
-                    q_key[lane] = ~0ull;
-                    const unsigned long long act = __ballot(1);
-                    const unsigned n_act = (unsigned)__popcll(act);
-                    const unsigned rank = (unsigned)__popcll(act & lanemask_lt());
-                    for (unsigned base = 0; base < qn; base += n_act) {
-                        const unsigned k = base + rank;
-                        const bool have = k < qn;
-                        const unsigned long long e = have ? q_test[k] : (unsigned long long)lane;
-                        const int owner = (int)(unsigned)(e & 63ull);
-                        const float3 qo = make_float3(__shfl(o.x, owner), __shfl(o.y, owner), __shfl(o.z, owner));
-                        const float3 qd = make_float3(__shfl(d.x, owner), __shfl(d.y, owner), __shfl(d.z, owner));
-                        const float qtmin = __shfl(tmin, owner), qtmax = __shfl(tmax, owner);
-                        const int qhit = __shfl(hit_addr, owner);
-                        prof(have, pr_tr, pr_tru);
-                        // batches: the owner's scene
-                        const float4* qtris = tris;
-                        if (MULTI) {
-                            const unsigned long long tp = reinterpret_cast<unsigned long long>(tris);
-                            const unsigned lo32 = (unsigned)__shfl((int)(unsigned)tp, owner);
-                            const unsigned hi32 = (unsigned)__shfl((int)(unsigned)(tp >> 32), owner);
-                            qtris = reinterpret_cast<const float4*>(((unsigned long long)hi32 << 32) | lo32);
-                        }
-                        if (have) {
-                            const int addr = (int)(unsigned)(e >> 32);
-                            const float4 v0 = qtris[addr], v1 = qtris[addr + 1], v2 = qtris[addr + 2];
-                            if (STATS) ++st_tris;
-                            float t, u, v;
-                            if (woop_test<ANY>(v0, v1, v2, qo, qd, qtmin, qtmax, addr, qhit, t, u, v)) {
-                                unsigned long long key;
-                                if (ANY) {
-                                    key = ((unsigned long long)k << 32) | (unsigned)addr;
-                                } else {
-                                    // order of t as an unsigned word (-0 taken as +0, as
-                                    // the comparisons of the tie rule see them), then address
-                                    const unsigned tb = f2u(t == 0.0f ? 0.0f : t);
-                                    const unsigned ord = (tb & 0x80000000u) ? ~tb : (tb | 0x80000000u);
-                                    key = ((unsigned long long)ord << 32) | (unsigned)addr;
-                                }
-                                __hip_atomic_fetch_min(&q_key[owner], key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                                // the best test of the owner so far leaves its result
-                                if (q_key[owner] == key) {
-                                    q_res[4 * owner] = t;
-                                    q_res[4 * owner + 1] = u;
-                                    q_res[4 * owner + 2] = v;
-                                }
-                            }
-                        }
-                    }
-                    const unsigned long long best = q_key[lane];
-                    if (best != ~0ull) {
-                        tmax = q_res[4 * lane];
-                        hit_addr = (int)(unsigned)(best & 0xFFFFFFFFull);
-                        hu = q_res[4 * lane + 1];
-                        hv = q_res[4 * lane + 2];
-                        if (ANY) finished = true;
-                    }
-                    mine = 0;
-                    qn = 0u;
-                }
-                if (lane == __ffsll((long long)__ballot(1)) - 1) *q_count = qn;
-                if (!finished && ng.y <= 0x00FFFFFFu && tg.y == 0u && sp == 0 && mine == 0) finished = true;
             } else {
                 // node step: lanes without triangles (and, speculating, every
                 // lane with inner children left) visit one node
@@ -948,8 +805,7 @@ int resident_blocks(K kernel, int block, size_t lds) {
 template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer, bool W7 = false>
 int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t stream) {
     const int block = cfg.block_size;
-    const size_t lds = (size_t)block * (size_t)cfg.lds_depth * sizeof(unsigned long long) +
-                       (LOOP == kLoopQueue ? (size_t)(block / 64) * kQueueBytesPerWave : 0);
+    const size_t lds = (size_t)block * (size_t)cfg.lds_depth * sizeof(unsigned long long);
     auto kernel = cwbvh_trace<ANY, STATS, MULTI, PREFETCH, LOOP, SRC, W7>;
     long long grid = cfg.grid_blocks;
     if (!MULTI && !p.persistent) {
@@ -1052,8 +908,7 @@ int32_t validate_config(int32_t kind, const cwbvh_trace_config& c) {
         set_error("lds_depth must be in [1, stack capacity]");
         return CWBVH_ERR_INVALID_ARGUMENT;
     }
-    size_t lds = (size_t)c.block_size * c.lds_depth * (kind == CWBVH_KIND_CWBVH ? 8 : 4);
-    if (kind == CWBVH_KIND_CWBVH && c.loop_mode == kLoopQueue) lds += (size_t)(c.block_size / 64) * kQueueBytesPerWave;
+    const size_t lds = (size_t)c.block_size * c.lds_depth * (kind == CWBVH_KIND_CWBVH ? 8 : 4);
     if (lds > 160 * 1024) {
         set_error("LDS stack exceeds 160 KiB per block");
         return CWBVH_ERR_INVALID_ARGUMENT;

@@ -352,7 +352,7 @@ typedef struct cwbvh_trace_config {
                                 per iteration (reference order), 1 = node visit or one triangle test
                                 per iteration, 2 = as 1, and lanes holding triangles also visit a
                                 node, parking its triangle group on the stack; 3 (per-wave triangle
-                                queue) is switched off and refused with CWBVH_ERR_INVALID_ARGUMENT */
+                                queue) was taken out and is refused with CWBVH_ERR_INVALID_ARGUMENT */
     int32_t occupancy;       /* register budget of the single-scene loop_mode 1 kernels (no prefetch,
                                 no stats; rays from a buffer or from surfels): 0 = automatic
                                 (7 waves per SIMD for launches of 2^22 rays or more), 1 = unconstrained,

@@ -123,7 +123,7 @@ def assert_any_hit_valid(orc, lay, rays, g, ref, what):
         assert ok and np.float32(t) == g[i, 0] and np.float32(u) == g[i, 2] and np.float32(v) == g[i, 3], (what, i)
 
 
-QUEUE_OFF = ("loop_mode 3 is switched off: a GPU run that included its first launches faulted the device "
+QUEUE_OFF = ("loop_mode 3 was taken out: a GPU run that included its first launches faulted the device "
              "together with another untested path, and no cause has been shown")
 
 
