@@ -71,6 +71,7 @@ struct CwbvhParams {
     int total_waves;
     int total_blocks;
     int spec_limit;                 // single scene: speculation_limit of its tree
+    int lds_only;                   // host side: the tree's stack need fits lds_depth (LDS-only kernels)
     // SRC = kRaysFromSurfels: generate ray r from surfel r / rays_per_texel
     const float4* __restrict__ surf_pos;
     const float4* __restrict__ surf_nrm;
@@ -166,10 +167,15 @@ constexpr long long kOccupancy7MinRays = 1ll << 22;  // automatic choice: launch
 #define CWBVH_TRACE_ATTR __attribute__((amdgpu_waves_per_eu(W7 ? 7 : 1)))
 #endif
 
-template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer, bool W7 = false>
+template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer, bool W7 = false,
+          bool LDSO = false>
 __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhParams P) {
     static_assert(!W7 || (LOOP == kLoopInterleave && !MULTI && !STATS && !PREFETCH),
                   "the 7-wave build is made for the plain interleaved kernels only");
+    // LDSO: the launch's tree needs at most lds_depth stack entries, so the
+    // stack lives in LDS only (interleaved schedule: only node groups are
+    // pushed, one per level of the tree)
+    static_assert(!LDSO || (LOOP == kLoopInterleave && !MULTI && !STATS), "LDS-only stacks serve single trees");
     extern __shared__ unsigned long long lds_stack_raw[];
     CWD_LDS unsigned long long* lds_stack = lds_ptr(lds_stack_raw);
     unsigned long long spill[kStackCapacity];
@@ -376,12 +382,25 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
 
         // stack push of a node or triangle group (overflow is flagged, not written)
         auto push = [&](uint2 e) {
-            if (sp < kStackCapacity) {
-                stack_put(lds_stack, spill, sx, sp, pack2(e));
-                ++sp;
+            if constexpr (LDSO) {
+                if (sp < P.lds_depth) {
+                    lds_stack[(sp << 6) + sx.col] = pack2(e);
+                    ++sp;
+                } else {
+                    P.work[kWorkOverflow] = 1u;
+                }
             } else {
-                P.work[kWorkOverflow] = 1u;
+                if (sp < kStackCapacity) {
+                    stack_put(lds_stack, spill, sx, sp, pack2(e));
+                    ++sp;
+                } else {
+                    P.work[kWorkOverflow] = 1u;
+                }
             }
+        };
+        auto stack_at = [&](int i) -> unsigned long long {
+            if constexpr (LDSO) return lds_stack[(i << 6) + sx.col];
+            else return stack_get(lds_stack, spill, sx, i);
         };
         // Visit the highest-priority child of ng: the remainder of ng goes on
         // the stack, ng becomes the child's inner children, the child's leaf
@@ -493,7 +512,7 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                 if (!finished && ng.y <= 0x00FFFFFFu) {
                     if (sp > 0) {
                         --sp;
-                        ng = unpack2(stack_get(lds_stack, spill, sx, sp));
+                        ng = unpack2(stack_at(sp));
                     } else {
                         finished = true;
                     }
@@ -529,7 +548,7 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                     if (tg.y == 0u) {
                         if (sp > 0) {
                             --sp;
-                            const uint2 e = unpack2(stack_get(lds_stack, spill, sx, sp));
+                            const uint2 e = unpack2(stack_at(sp));
                             if (e.y > 0x00FFFFFFu) {
                                 ng = e;
                             } else {
@@ -540,7 +559,7 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                             finished = true;
                         }
                     } else if (LOOP == kLoopSpeculate && sp > 0) {
-                        const uint2 e = unpack2(stack_get(lds_stack, spill, sx, sp - 1));
+                        const uint2 e = unpack2(stack_at(sp - 1));
                         if (e.y > 0x00FFFFFFu) {
                             --sp;
                             ng = e;
@@ -802,11 +821,12 @@ int resident_blocks(K kernel, int block, size_t lds) {
     return nb;
 }
 
-template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer, bool W7 = false>
+template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer, bool W7 = false,
+          bool LDSO = false>
 int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t stream) {
     const int block = cfg.block_size;
     const size_t lds = (size_t)block * (size_t)cfg.lds_depth * sizeof(unsigned long long);
-    auto kernel = cwbvh_trace<ANY, STATS, MULTI, PREFETCH, LOOP, SRC, W7>;
+    auto kernel = cwbvh_trace<ANY, STATS, MULTI, PREFETCH, LOOP, SRC, W7, LDSO>;
     long long grid = cfg.grid_blocks;
     if (!MULTI && !p.persistent) {
         grid = ((long long)p.n + block - 1) / block;
@@ -841,9 +861,13 @@ template <bool ANY, bool STATS, bool MULTI, bool PREFETCH>
 int32_t dispatch_loop(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t st) {
     switch (cfg.loop_mode) {
         case kLoopInterleave:
-            if constexpr (!STATS && !MULTI && !PREFETCH) {
-                const bool w7 = cfg.occupancy == kOccupancy7 ||
-                                (cfg.occupancy == kOccupancyAuto && (long long)p.n >= kOccupancy7MinRays);
+            if constexpr (!STATS && !MULTI) {
+                const bool w7 = !PREFETCH && (cfg.occupancy == kOccupancy7 ||
+                                              (cfg.occupancy == kOccupancyAuto && (long long)p.n >= kOccupancy7MinRays));
+                if (p.lds_only) {
+                    if (w7) return launch_cwbvh_t<ANY, false, false, false, kLoopInterleave, kRaysFromBuffer, true, true>(p, cfg, st);
+                    return launch_cwbvh_t<ANY, false, false, PREFETCH, kLoopInterleave, kRaysFromBuffer, false, true>(p, cfg, st);
+                }
                 if (w7) return launch_cwbvh_t<ANY, false, false, false, kLoopInterleave, kRaysFromBuffer, true>(p, cfg, st);
             }
             return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopInterleave>(p, cfg, st);
@@ -872,6 +896,10 @@ int32_t dispatch_surfels(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStrea
             if constexpr (!STATS) {
                 const bool w7 = cfg.occupancy == kOccupancy7 ||
                                 (cfg.occupancy == kOccupancyAuto && (long long)p.n >= kOccupancy7MinRays);
+                if (p.lds_only) {
+                    if (w7) return launch_cwbvh_t<ANY, false, false, false, kLoopInterleave, kRaysFromSurfels, true, true>(p, cfg, st);
+                    return launch_cwbvh_t<ANY, false, false, false, kLoopInterleave, kRaysFromSurfels, false, true>(p, cfg, st);
+                }
                 if (w7) return launch_cwbvh_t<ANY, false, false, false, kLoopInterleave, kRaysFromSurfels, true>(p, cfg, st);
             }
             return launch_cwbvh_t<ANY, STATS, false, false, kLoopInterleave, kRaysFromSurfels>(p, cfg, st);
@@ -939,6 +967,13 @@ int32_t validate_config(int32_t kind, const cwbvh_trace_config& c) {
     return CWBVH_OK;
 }
 
+// A tree whose stack need (known from its layout, or declared when caller
+// arrays are wrapped) fits the LDS part of the stack is traversed by kernels
+// without the scratch part.
+static bool lds_only(const cwbvh_scene* s, const cwbvh_trace_config& c) {
+    return s->stack_need > 0 && s->stack_need <= c.lds_depth && c.lds_depth <= kStackCapacity;
+}
+
 int32_t launch_cwbvh(const TraceLaunch& L) {
     const cwbvh_scene* s = L.scene;
     CwbvhParams p{};
@@ -966,6 +1001,7 @@ int32_t launch_cwbvh(const TraceLaunch& L) {
     // 262k rays trace 2x faster one per thread, 1M rays 5% slower)
     p.persistent = L.cfg.persistent == 2 ? (L.n >= cw::kAutoPersistentRays ? 1 : 0) : L.cfg.persistent;
     p.spec_limit = cw::speculation_limit(s->stack_need);
+    p.lds_only = lds_only(s, L.cfg) ? 1 : 0;
     return dispatch_cwbvh<false>(p, L.flags, L.cfg, L.stream);
 }
 
@@ -995,6 +1031,7 @@ int32_t launch_cwbvh_surfels(const TraceLaunch& L) {
     p.slices = L.cfg.work_slices;
     p.persistent = L.cfg.persistent == 2 ? (L.n >= cw::kAutoPersistentRays ? 1 : 0) : L.cfg.persistent;
     p.spec_limit = cw::speculation_limit(s->stack_need);
+    p.lds_only = lds_only(s, L.cfg) ? 1 : 0;
     p.surf_pos = (const float4*)L.surf_pos;
     p.surf_nrm = (const float4*)L.surf_nrm;
     p.surf_rad = L.surf_rad;
