@@ -476,6 +476,10 @@ int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_confi
     // quantize origins; its unsorted rays keep the grouped schedule, which
     // tests a leaf's triangles right after fetching the node (C3 shuffled:
     // 1.10x over interleaved).
+    // a tree whose stack need exceeds the default LDS depth keeps its whole
+    // stack in LDS (the kernels without the scratch part serve it)
+    if (scene->kind == CWBVH_KIND_CWBVH && scene->stack_need > c->lds_depth && scene->stack_need <= CWBVH_STACK_CAPACITY)
+        c->lds_depth = scene->stack_need;
     if (scene->bytes > kCacheResidentBytes) {
         if (scene->bounds_known) c->ray_order = 1;
         if (scene->kind == CWBVH_KIND_CWBVH) {

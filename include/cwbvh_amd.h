@@ -375,7 +375,8 @@ int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
  * interleaved schedule with node prefetch (loop_mode 1, prefetch 1); one
  * whose root holds no child box gets the grouped schedule (loop_mode 0).
  * When the pre-pass cannot allocate its workspace, the launch traces the rays
- * in the given order. */
+ * in the given order.  A CWBVH tree whose stack need exceeds the default LDS
+ * depth gets lds_depth = its stack need, so its whole stack stays in LDS. */
 int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_config* cfg);
 
 /* Traverse n rays.  d_counters (nullable, CWBVH_TRACE_STATS only) receives
