@@ -452,14 +452,15 @@ def bench_c2(args, cw, W, dev) -> dict:
     ms_plain = timed(plain, args.steps, args.warmup)
     log(f"plain {ms_plain:.2f} ms")
     t = time.perf_counter()
-    best, recs = cw.autotune_batch(items, block_sizes=(64, 128, 256), lds_depths=(8, 16, 32), morton=(0, 1),
+    best, recs = cw.autotune_batch(items, block_sizes=(64, 128, 256), lds_depths=(0, 8, 16, 32), morton=(0, 1),
                                    prefetch=(0, 1), loop_modes=(0, 1, 2), reps=args.tune_reps, stream=st)
     tune_s = time.perf_counter() - t
     log(f"autotune {tune_s:.1f} s best {best.as_dict()}")
     batch = cw.Batch(items, cw.TRACE_CLOSEST, best.cfg, morton=bool(best.morton))
     batch.capture(st)
     ms_batch = timed(lambda: batch.replay(st), args.steps, args.warmup)
-    untuned = cw.Batch(items, cw.TRACE_CLOSEST, cfg0, morton=False)
+    # the library's batch default (no configuration given)
+    untuned = cw.Batch(items, cw.TRACE_CLOSEST, None, morton=False)
     ms_untuned = timed(lambda: untuned.launch(st), args.steps, args.warmup)
     value = total / (ms_batch / 1e3) / 1e6
 

@@ -919,10 +919,11 @@ class Batch:
             self._b = C.c_void_p(0)
 
 
-def autotune_batch(items, flags: int = TRACE_CLOSEST, block_sizes=(64, 128, 256), lds_depths=(8, 16, 32),
+def autotune_batch(items, flags: int = TRACE_CLOSEST, block_sizes=(64, 128, 256), lds_depths=(0, 8, 16, 32),
                    morton=(0, 1), prefetch=(0,), loop_modes=(), reps: int = 3, stream: Stream | None = None):
     """Time every configuration of the space on the batch; returns (best, records).
-    An empty loop_modes keeps the default inner-loop schedule."""
+    An empty loop_modes keeps the default inner-loop schedule.  An LDS depth
+    of 0 stands for the deepest stack need of the items' trees."""
     sp = TuneSpace()
     for name, vals, lo, cap in (("block_sizes", block_sizes, 1, 4), ("lds_depths", lds_depths, 1, 4),
                                 ("morton", morton, 1, 2), ("prefetch", prefetch, 1, 2), ("loop_modes", loop_modes, 0, 3)):

@@ -123,6 +123,8 @@ struct cwbvh_batch {
     void* d_sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     int key_bits = cw::kRayKeyBits;
+    // deepest stack need over the jobs' trees; -1 when a tree's is unknown
+    int32_t max_need = 0;
     int64_t bytes = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -216,6 +218,7 @@ int32_t batch_trace(cwbvh_batch* b, hipStream_t st) {
     L.stats = nullptr;
     L.counters = b->counters;
     L.stream = st;
+    L.lds_only = b->max_need > 0 && b->max_need <= b->cfg.lds_depth && b->cfg.lds_depth <= CWBVH_STACK_CAPACITY;
     return cw::launch_cwbvh_batch(L);
 }
 
@@ -338,6 +341,8 @@ int32_t cwbvh_batch_create(const cwbvh_batch_item* items, int32_t count, uint32_
         J.node_stride = it.scene->node_stride;
         J.chunk_begin = (uint32_t)chunks;
         J.spec_limit = speculation_limit(it.scene->stack_need);
+        if (it.scene->stack_need <= 0) b->max_need = -1;
+        else if (b->max_need >= 0) b->max_need = std::max(b->max_need, it.scene->stack_need);
         chunks += ((uint64_t)it.n + c.fetch_chunk - 1) / c.fetch_chunk;
         if (chunks > 0xFFFFFFF0ull) {
             delete b;
@@ -359,6 +364,9 @@ int32_t cwbvh_batch_create(const cwbvh_batch_item* items, int32_t count, uint32_
         b->jobs.push_back(J);
     }
     b->total_chunks = (uint32_t)chunks;
+    // as the scene default does: with the default configuration, trees
+    // deeper than its LDS depth keep their whole stack in LDS
+    if (!cfg && b->max_need > b->cfg.lds_depth && b->max_need <= CWBVH_STACK_CAPACITY) b->cfg.lds_depth = b->max_need;
     if (b->morton && b->total_rays > 0x7FFFFFFF) {
         delete b;
         set_error("Morton pre-pass supports < 2^31 rays per batch");
@@ -478,6 +486,16 @@ int32_t cwbvh_autotune_batch(const cwbvh_batch_item* items, int32_t count, uint3
     hipEvent_t e0 = nullptr, e1 = nullptr;
     CW_HIP(hipEventCreate(&e0));
     CW_HIP(hipEventCreate(&e1));
+    // lds_depths entry 0: the deepest stack need of the items' trees
+    int32_t need = 0;
+    for (int32_t i = 0; i < count && items; ++i) {
+        const cwbvh_scene* sc = items[i].scene;
+        if (!sc || sc->stack_need <= 0) {
+            need = -1;
+            break;
+        }
+        need = std::max(need, sc->stack_need);
+    }
     int32_t n = 0;
     best->ms = INFINITY;
     int32_t rc = CWBVH_OK;
@@ -490,6 +508,7 @@ int32_t cwbvh_autotune_batch(const cwbvh_batch_item* items, int32_t count, uint3
                     cwbvh_trace_config_default(CWBVH_KIND_CWBVH, &c);
                     c.block_size = space->block_sizes[ib];
                     c.lds_depth = space->lds_depths[il];
+                    if (c.lds_depth == 0) c.lds_depth = need > 0 && need <= CWBVH_STACK_CAPACITY ? need : 10;
                     c.prefetch = space->prefetch[ip];
                     if (space->n_loop_modes > 0) c.loop_mode = space->loop_modes[io];
                     cwbvh_batch* b = nullptr;

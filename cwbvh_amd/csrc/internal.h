@@ -151,6 +151,7 @@ struct BatchLaunch {
     uint64_t* stats;
     unsigned* counters;
     hipStream_t stream;
+    int32_t lds_only;         // every job's tree needs at most cfg.lds_depth stack entries
 };
 // Rays per job chunk of a multi-scene launch when the configuration asks for
 // an automatic chunk size (fetch_chunk == 0); batch plans and the kernel must

@@ -212,6 +212,7 @@ int32_t cwbvh_server::launch(Slot& sl, const std::vector<Request*>& batch) {
     }
     const int32_t chunk = cfg.trace.fetch_chunk;
     uint32_t chunks = 0;
+    int32_t max_need = 0;  // deepest stack need of the launch's trees (-1: unknown)
     for (size_t k = 0; k < batch.size(); ++k) {
         const Request* r = batch[k];
         const cwbvh_scene* sc = scenes[r->scene];
@@ -234,6 +235,8 @@ int32_t cwbvh_server::launch(Slot& sl, const std::vector<Request*>& batch) {
         J.chunk_begin = chunks;
         J.spec_limit = speculation_limit(sc->stack_need);
         chunks += (uint32_t)((r->n + chunk - 1) / chunk);
+        if (sc->stack_need <= 0) max_need = -1;
+        else if (max_need >= 0) max_need = std::max(max_need, sc->stack_need);
     }
     if (cfg.serial) {
         const Request* r = batch[0];
@@ -259,6 +262,7 @@ int32_t cwbvh_server::launch(Slot& sl, const std::vector<Request*>& batch) {
         L.cfg = cfg.trace;
         L.counters = sl.counters;
         L.stream = sl.stream;
+        L.lds_only = max_need > 0 && max_need <= cfg.trace.lds_depth && cfg.trace.lds_depth <= CWBVH_STACK_CAPACITY;
         int32_t rc = launch_cwbvh_batch(L);
         if (rc) return rc;
     }

@@ -172,10 +172,10 @@ template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = k
 __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhParams P) {
     static_assert(!W7 || (LOOP == kLoopInterleave && !MULTI && !STATS && !PREFETCH),
                   "the 7-wave build is made for the plain interleaved kernels only");
-    // LDSO: the launch's tree needs at most lds_depth stack entries, so the
-    // stack lives in LDS only (interleaved schedule: only node groups are
-    // pushed, one per level of the tree)
-    static_assert(!LDSO || (LOOP == kLoopInterleave && !MULTI && !STATS), "LDS-only stacks serve single trees");
+    // LDSO: the launch's trees (a batch: every job's tree) need at most
+    // lds_depth stack entries, so the stack lives in LDS only (interleaved
+    // schedule: only node groups are pushed, one per level of the tree)
+    static_assert(!LDSO || (LOOP == kLoopInterleave && !STATS), "LDS-only stacks serve the interleaved schedule");
     extern __shared__ unsigned long long lds_stack_raw[];
     CWD_LDS unsigned long long* lds_stack = lds_ptr(lds_stack_raw);
     unsigned long long spill[kStackCapacity];
@@ -625,6 +625,12 @@ struct Bvh2Params {
     int lds_depth;
     int woop_last;   // last valid float4 index of tris (loads past a leaf terminator clamp here)
     int prim_id;
+    // persistent launches (bvh2_trace_persistent) only
+    int slices;
+    int chunk;
+    int idle_lanes;
+    int idle_budget;
+    int total_blocks;
 };
 
 // Slab test of one BVH2 child box (reference src/TraversalKernelBVH2.cu:14-24).
@@ -738,6 +744,196 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace(Bvh2Params P) {
     }
 }
 
+// Persistent BVH2 traversal.  One thread per ray leaves a lane idle from the
+// moment its ray ends until the slowest ray of its wave ends (C1 AO rays:
+// 34% VALU lane use).  Here a grid sized to the device keeps its waves and
+// hands rays to idle lanes: rays are claimed `chunk` at a time from `slices`
+// contiguous ranges (a wave starts with the range of its XCD) and given to the
+// idle lanes by ballot rank.  Each lane runs the while-while rounds of
+// bvh2_trace on its ray unchanged (a round: node steps until every lane of the
+// wave holds a leaf, then the leaves), so each ray's visits and tests, and its
+// hit, are the same.  After each round a wave counts its idle lanes; once it
+// has lost `idle_budget` lane-rounds beyond `idle_lanes` per round, it leaves
+// the rounds to refill.  The last block to exit resets the claim counters.
+template <bool ANY, bool STATS>
+__global__ __launch_bounds__(kMaxBlock) void bvh2_trace_persistent(Bvh2Params P) {
+    constexpr int SENTINEL = 0x76543210;
+    extern __shared__ int lds_stack_raw_i[];
+    CWD_LDS int* lds_stack_i = lds_ptr(lds_stack_raw_i);
+    int spill[kBvh2StackCapacity];
+    const StackIndex sx((int)threadIdx.x, P.lds_depth);
+    const int lane = lane_id();
+
+    int out_idx = 0;
+    float3 o = make_float3(0.f, 0.f, 0.f), d = o;
+    float tmin = 0.f, hitT = 0.f, idx = 0.f, idy = 0.f, idz = 0.f, oodx = 0.f, oody = 0.f, oodz = 0.f;
+    int sp = 0, node = SENTINEL, leaf = 0, hit_addr = -1;
+    float hu = 0.f, hv = 0.f;
+    bool busy = false;
+    unsigned long long st_nodes = 0, st_tris = 0;
+
+    // wave-uniform claim state
+    unsigned pool_next = 0, pool_end = 0;
+    bool exhausted = false;
+    const int home = (int)(blockIdx.x % (unsigned)P.slices);
+    int slice_k = 0;
+
+    while (true) {
+        const bool idle = !busy;
+        unsigned long long need = __ballot(idle);
+        bool got = false;
+        unsigned my = 0;
+        while (need != 0ull) {
+            if (pool_next >= pool_end) {
+                if (exhausted) break;
+                const int leader = __ffsll((long long)need) - 1;
+                bool found = false;
+                while (slice_k < P.slices) {
+                    const int sl = (home + slice_k) % P.slices;
+                    const unsigned lo = (unsigned)(((unsigned long long)P.n * (unsigned)sl) / (unsigned)P.slices);
+                    const unsigned hi = (unsigned)(((unsigned long long)P.n * (unsigned)(sl + 1)) / (unsigned)P.slices);
+                    unsigned base = 0;
+                    if (lane == leader) base = atomicAdd(&P.work[sl * kWorkStride], (unsigned)P.chunk);
+                    base = __shfl(base, leader);
+                    if (base < hi - lo) {
+                        pool_next = lo + base;
+                        pool_end = lo + min(base + (unsigned)P.chunk, hi - lo);
+                        found = true;
+                        break;
+                    }
+                    ++slice_k;
+                }
+                if (!found) {
+                    exhausted = true;
+                    break;
+                }
+            }
+            const unsigned avail = pool_end - pool_next;
+            const unsigned rank = (unsigned)__popcll(need & lanemask_lt());
+            if (((need >> lane) & 1ull) && rank < avail) {
+                my = pool_next + rank;
+                got = true;
+            }
+            pool_next += min((unsigned)__popcll(need), avail);
+            need = __ballot(idle && !got);
+        }
+        if (idle && !got) break;  // no rays left for this lane
+
+        if (got) {
+            int ri = P.order ? P.order[my] : (int)my;
+            if ((unsigned)ri >= (unsigned)P.n) ri = (int)my;  // malformed order: stay in bounds
+            out_idx = ri;
+            const float4 ro = nt_load4(&P.rays[ri].o);
+            const float4 rd = nt_load4(&P.rays[ri].d);
+            o = make_float3(ro.x, ro.y, ro.z);
+            d = make_float3(rd.x, rd.y, rd.z);
+            tmin = ro.w;
+            hitT = rd.w;
+            idx = safe_rcp(d.x);
+            idy = safe_rcp(d.y);
+            idz = safe_rcp(d.z);
+            oodx = o.x * idx;
+            oody = o.y * idy;
+            oodz = o.z * idz;
+            sp = 0;
+            stack_put(lds_stack_i, spill, sx, 0, SENTINEL);
+            node = 0;
+            hit_addr = -1;
+            hu = hv = 0.f;
+            busy = true;
+        }
+
+        int lost = 0;
+        while (true) {
+            bool done = false;
+            leaf = 0;
+            while (node != SENTINEL && node >= 0) {
+                const float4* np = P.nodes + node;
+                const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+                if (STATS) ++st_nodes;
+                float d0, d1;
+                const bool t0 = box2(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, idx, idy, idz, oodx, oody, oodz, tmin, hitT, d0);
+                const bool t1 = box2(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, idx, idy, idz, oodx, oody, oodz, tmin, hitT, d1);
+                const int c0 = f2i(n3.x), c1 = f2i(n3.y);
+                if (!t0 && !t1) {
+                    node = stack_get(lds_stack_i, spill, sx, sp);
+                    --sp;
+                } else {
+                    node = t0 ? c0 : c1;
+                    if (t0 && t1) {
+                        int far = c1;
+                        if (d1 < d0) {
+                            far = node;
+                            node = c1;
+                        }
+                        if (sp + 1 < kBvh2StackCapacity) {
+                            ++sp;
+                            stack_put(lds_stack_i, spill, sx, sp, far);
+                        } else {
+                            P.work[kWorkOverflow] = 1u;
+                        }
+                    }
+                }
+                if (node < 0 && leaf >= 0) {
+                    leaf = node;
+                    node = stack_get(lds_stack_i, spill, sx, sp);
+                    --sp;
+                }
+                if (!__any(leaf >= 0)) break;
+            }
+            while (leaf < 0) {
+                for (int a = ~leaf;; a += 3) {
+                    const float4 v0 = P.tris[a];
+                    const float4 v1 = P.tris[min(a + 1, P.woop_last)];
+                    const float4 v2 = P.tris[min(a + 2, P.woop_last)];
+                    if (f2u(v0.x) == 0x80000000u) break;
+                    if (STATS) ++st_tris;
+                    float t, u, v;
+                    if (woop_test<ANY>(v0, v1, v2, o, d, tmin, hitT, a, hit_addr, t, u, v)) {
+                        hitT = t;
+                        hit_addr = a;
+                        hu = u;
+                        hv = v;
+                        if (ANY) {
+                            done = true;
+                            break;
+                        }
+                    }
+                }
+                if (done) break;
+                leaf = node;
+                if (node < 0) {
+                    node = stack_get(lds_stack_i, spill, sx, sp);
+                    --sp;
+                }
+            }
+            if (done || node == SENTINEL) {
+                const int out_id = (hit_addr >= 0 && P.prim_id) ? P.prim_map[hit_addr] : hit_addr;
+                nt_store4(make_float4(hitT, i2f(out_id), hu, hv), &P.hits[out_idx]);
+                node = SENTINEL;
+                busy = false;
+                break;
+            }
+            lost += __popcll(~__ballot(1)) - P.idle_lanes;
+            if (lost >= P.idle_budget) break;
+        }
+    }
+    if (STATS) {
+        atomicAdd(&P.stats[0], st_nodes);
+        atomicAdd(&P.stats[1], st_tris);
+    }
+    // every wave has consumed the result of its last claim before this
+    // barrier, so the claims are complete when the exit is counted
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned prev = atomicAdd(&P.work[kWorkExited], 1u);
+        if (prev == (unsigned)P.total_blocks - 1u) {
+            for (int k = 0; k < kWorkSlices; ++k) atomicExch(&P.work[k * kWorkStride], 0u);
+            atomicExch(&P.work[kWorkExited], 0u);
+        }
+    }
+}
+
 __global__ void copy_u4_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, long long n) {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long step = (long long)gridDim.x * blockDim.x;
@@ -848,11 +1044,31 @@ int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_
 }
 
 template <bool ANY, bool STATS>
-int32_t launch_bvh2_t(Bvh2Params& p, const cwbvh_trace_config& cfg, hipStream_t stream) {
+int32_t launch_bvh2_t(Bvh2Params& p, const cwbvh_trace_config& cfg, bool persistent, hipStream_t stream) {
     const int block = cfg.block_size;
     const size_t lds = (size_t)block * (size_t)cfg.lds_depth * sizeof(int);
-    const long long grid = ((long long)p.n + block - 1) / block;
-    hipLaunchKernelGGL((bvh2_trace<ANY, STATS>), dim3((unsigned)grid), dim3(block), lds, stream, p);
+    if (!persistent) {
+        const long long grid = ((long long)p.n + block - 1) / block;
+        hipLaunchKernelGGL((bvh2_trace<ANY, STATS>), dim3((unsigned)grid), dim3(block), lds, stream, p);
+        CW_HIP(hipGetLastError());
+        return CWBVH_OK;
+    }
+    auto kernel = bvh2_trace_persistent<ANY, STATS>;
+    long long grid = cfg.grid_blocks;
+    if (grid <= 0) grid = (long long)resident_blocks(kernel, block, lds) * cw::device_compute_units();
+    p.total_blocks = (int)grid;
+    p.slices = cfg.work_slices;
+    p.idle_lanes = cfg.refill_idle;
+    p.idle_budget = cfg.refill_budget;
+    p.chunk = cfg.fetch_chunk;
+    if (p.chunk <= 0) {
+        // automatic: about 32 claims per wave over the launch, 128..512 rays each
+        const long long waves = grid * (block / 64);
+        long long c = (long long)p.n / (waves * 32);
+        c = std::min<long long>(512, std::max<long long>(128, (c + 63) / 64 * 64));
+        p.chunk = (int)c;
+    }
+    hipLaunchKernelGGL(kernel, dim3((unsigned)grid), dim3(block), lds, stream, p);
     CW_HIP(hipGetLastError());
     return CWBVH_OK;
 }
@@ -869,6 +1085,10 @@ int32_t dispatch_loop(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t
                     return launch_cwbvh_t<ANY, false, false, PREFETCH, kLoopInterleave, kRaysFromBuffer, false, true>(p, cfg, st);
                 }
                 if (w7) return launch_cwbvh_t<ANY, false, false, false, kLoopInterleave, kRaysFromBuffer, true>(p, cfg, st);
+            }
+            if constexpr (!STATS && MULTI) {
+                if (p.lds_only)
+                    return launch_cwbvh_t<ANY, false, true, PREFETCH, kLoopInterleave, kRaysFromBuffer, false, true>(p, cfg, st);
             }
             return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopInterleave>(p, cfg, st);
         case kLoopSpeculate: return launch_cwbvh_t<ANY, STATS, MULTI, PREFETCH, kLoopSpeculate>(p, cfg, st);
@@ -961,6 +1181,23 @@ int32_t validate_config(int32_t kind, const cwbvh_trace_config& c) {
         }
         if (c.postpone_pct > 0 && c.loop_mode == kLoopInterleave) {
             set_error("postpone_pct applies to loop_mode 0 and 2");
+            return CWBVH_ERR_INVALID_ARGUMENT;
+        }
+    } else {
+        // BVH2: one ray per thread (persistent 0) or the persistent grid
+        // (1, or 2 = automatic by ray count), whose claim and refill fields
+        // then apply; the CWBVH-only fields must stay 0
+        if (c.persistent < 0 || c.persistent > 2 || c.loop_mode != 0 || c.prefetch != 0 || c.postpone_pct != 0 ||
+            c.occupancy != 0) {
+            set_error("invalid BVH2 trace configuration: persistent is 0, 1 or 2; loop_mode, prefetch, "
+                      "postpone_pct and occupancy are 0");
+            return CWBVH_ERR_INVALID_ARGUMENT;
+        }
+        if (c.persistent != 0 && (c.fetch_chunk < 0 || c.fetch_chunk > (1 << 20) || c.grid_blocks < 0 ||
+                                  c.refill_budget < 1 || c.refill_idle < 0 || c.refill_idle > 64 ||
+                                  c.work_slices < 1 || c.work_slices > kWorkSlices)) {
+            set_error("invalid BVH2 persistent configuration (refill_budget >= 1, refill_idle in [0, 64], "
+                      "work_slices in [1, 8])");
             return CWBVH_ERR_INVALID_ARGUMENT;
         }
     }
@@ -1077,6 +1314,7 @@ int32_t launch_cwbvh_batch(const BatchLaunch& L) {
     p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
     p.slices = L.cfg.work_slices;
     p.persistent = 1;
+    p.lds_only = L.lds_only;
     return dispatch_cwbvh<true>(p, L.flags, L.cfg, L.stream);
 }
 
@@ -1095,9 +1333,11 @@ int32_t launch_bvh2(const TraceLaunch& L) {
     p.lds_depth = L.cfg.lds_depth;
     p.woop_last = s->woop_f4 > 0 ? (int)(s->woop_f4 - 1) : 0x7FFFFFFF;
     p.prim_id = (L.flags & CWBVH_TRACE_PRIM_ID) ? 1 : 0;
+    // persistent 2 = automatic: the persistent grid from kAutoPersistentRays up
+    const bool pers = L.cfg.persistent == 2 ? L.n >= cw::kAutoPersistentRays : L.cfg.persistent == 1;
     const bool any = (L.flags & CWBVH_TRACE_ANY) != 0, stats = (L.flags & CWBVH_TRACE_STATS) != 0;
-    if (any) return stats ? launch_bvh2_t<true, true>(p, L.cfg, L.stream) : launch_bvh2_t<true, false>(p, L.cfg, L.stream);
-    return stats ? launch_bvh2_t<false, true>(p, L.cfg, L.stream) : launch_bvh2_t<false, false>(p, L.cfg, L.stream);
+    if (any) return stats ? launch_bvh2_t<true, true>(p, L.cfg, pers, L.stream) : launch_bvh2_t<true, false>(p, L.cfg, pers, L.stream);
+    return stats ? launch_bvh2_t<false, true>(p, L.cfg, pers, L.stream) : launch_bvh2_t<false, false>(p, L.cfg, pers, L.stream);
 }
 
 }  // namespace cw

@@ -336,8 +336,8 @@ void cwbvh_scene_destroy(cwbvh_scene* scene);
 typedef struct cwbvh_trace_config {
     int32_t block_size;      /* 64, 128 or 256 threads */
     int32_t lds_depth;       /* stack entries per lane kept in LDS (1..stack capacity) */
-    int32_t grid_blocks;     /* persistent CWBVH grid (0 = fill the device) */
-    int32_t fetch_chunk;     /* rays claimed per wave-level atomic (CWBVH; 0 = automatic) */
+    int32_t grid_blocks;     /* persistent grid (0 = fill the device) */
+    int32_t fetch_chunk;     /* rays claimed per wave-level atomic (persistent grids; 0 = automatic) */
     int32_t refill_idle;     /* leave the inner loop once idle lanes exceed this ... */
     int32_t refill_budget;   /* ... for this many accumulated lane-iterations */
     int32_t postpone_pct;    /* 0 = off; loop_mode 0: postpone a triangle group when fewer than this %
@@ -345,9 +345,13 @@ typedef struct cwbvh_trace_config {
                                 this % of the active lanes hold triangles and no node to visit;
                                 not for loop_mode 1 */
     int32_t prefetch;        /* 0 or 1: issue the next node's loads before the triangle loop */
-    int32_t work_slices;     /* 1..8 contiguous ray ranges, one drained first per XCD (CWBVH) */
-    int32_t persistent;      /* CWBVH: 1 = persistent grid with work fetching, 0 = one ray per thread,
-                                2 = persistent from 2^19 rays up (default) */
+    int32_t work_slices;     /* 1..8 contiguous ray ranges, one drained first per XCD (persistent grids) */
+    int32_t persistent;      /* 1 = persistent grid with work fetching, 0 = one ray per thread,
+                                2 = persistent from 2^19 rays up.  CWBVH default 2.  BVH2 default 0
+                                (the reference's kernel); its persistent grid hands rays to idle lanes
+                                between while-while rounds and uses grid_blocks, fetch_chunk,
+                                refill_idle (>= 0), refill_budget (>= 1) and work_slices.  BVH2
+                                configurations keep loop_mode, prefetch, postpone_pct and occupancy 0 */
     int32_t loop_mode;       /* CWBVH inner-loop schedule: 0 = node visit + its whole triangle group
                                 per iteration (reference order), 1 = node visit or one triangle test
                                 per iteration, 2 = as 1, and lanes holding triangles also visit a
@@ -374,6 +378,8 @@ int32_t cwbvh_trace_config_default(int32_t kind, cwbvh_trace_config* cfg);
  * (ray_order 1, in bounds read from its root node) and, for CWBVH, the
  * interleaved schedule with node prefetch (loop_mode 1, prefetch 1); one
  * whose root holds no child box gets the grouped schedule (loop_mode 0).
+ * A large BVH2 scene gets the persistent grid (persistent 2, refill_idle 2,
+ * refill_budget 8, work_slices 8).
  * When the pre-pass cannot allocate its workspace, the launch traces the rays
  * in the given order.  A CWBVH tree whose stack need exceeds the default LDS
  * depth gets lds_depth = its stack need, so its whole stack stays in LDS. */
@@ -626,7 +632,9 @@ int32_t cwbvh_server_run_poisson_surfels(cwbvh_server* s, const cwbvh_poisson_sp
 typedef struct cwbvh_tune_space {
     int32_t block_sizes[4];
     int32_t n_block;
-    int32_t lds_depths[4];
+    int32_t lds_depths[4];     /* 0 = the deepest stack need of the items' trees (the
+                                  smallest depth at which the batch kernels keep the
+                                  whole stack in LDS); 10 when a tree's need is unknown */
     int32_t n_lds;
     int32_t morton[2];         /* 0 = rays as given, 1 = Morton pre-pass */
     int32_t n_morton;

@@ -85,7 +85,10 @@ def test_batch_matches_oracle(cw, batch_set, morton, any_hit):
                                 # the autotuner's usual picks on many-scene batches
                                 dict(loop_mode=2, block_size=256, lds_depth=8),
                                 dict(loop_mode=2, block_size=64, lds_depth=8),
-                                dict(loop_mode=1, block_size=64, lds_depth=8)])
+                                dict(loop_mode=1, block_size=64, lds_depth=8),
+                                # every tree's stack fits the LDS depth: the LDS-only batch kernels
+                                dict(loop_mode=1, block_size=128, lds_depth=64),
+                                dict(loop_mode=1, block_size=256, lds_depth=40, prefetch=1)])
 def test_batch_configs(cw, batch_set, kw):
     _, items, refs = batch_set
     bufs, ditems = device_items(cw, items)
@@ -225,3 +228,20 @@ def test_autotune_small(cw, batch_set):
     b.launch()
     cw.synchronize()
     check_items(cw, items, bufs, refs, False, "autotuned")
+
+
+def test_autotune_lds_depth_of_the_stack_need(cw, batch_set):
+    """An LDS depth of 0 in the tuning space stands for the deepest stack need
+    of the items' trees, the smallest depth at which the batch keeps the whole
+    stack in LDS; that configuration's hits equal the oracle's."""
+    _, items, refs = batch_set
+    bufs, ditems = device_items(cw, items)
+    need = max(lay.stats()["stack_need"] for _, lay, rays in items if len(rays))
+    say(f"autotune at the stack need ({need})")
+    best, recs = cw.autotune_batch(ditems, block_sizes=(128,), lds_depths=(0,), morton=(0,), prefetch=(0,),
+                                   loop_modes=(1,), reps=1)
+    assert len(recs) == 1 and recs[0].cfg.lds_depth == need and best.cfg.lds_depth == need
+    b = cw.Batch(ditems, cw.TRACE_CLOSEST, best.cfg)
+    b.launch()
+    cw.synchronize()
+    check_items(cw, items, bufs, refs, False, "LDS depth = stack need")

@@ -149,8 +149,14 @@ def test_cwbvh_config_variants_bit_identical(cw, orc, c0, c0_ref, variant):
         assert not cw.stack_overflowed(), what
 
 
-@pytest.mark.parametrize("kw", [dict(block_size=64, lds_depth=1), dict(block_size=256, lds_depth=8),
-                                dict(block_size=128, lds_depth=64)])
+@pytest.mark.parametrize("kw", [
+    dict(block_size=64, lds_depth=1), dict(block_size=256, lds_depth=8), dict(block_size=128, lds_depth=64),
+    # the persistent grid: each ray's visits and tests are those of one ray per thread
+    dict(persistent=1, refill_idle=4, refill_budget=16, work_slices=8),
+    dict(persistent=2, refill_idle=4, refill_budget=16, work_slices=8, block_size=128, lds_depth=4),
+    dict(persistent=1, refill_idle=0, refill_budget=1, work_slices=1, fetch_chunk=64, block_size=64, lds_depth=2),
+    dict(persistent=1, refill_idle=32, refill_budget=1000, work_slices=3, fetch_chunk=1000, grid_blocks=7),
+    dict(persistent=1, refill_idle=64, refill_budget=1, work_slices=8, fetch_chunk=1, grid_blocks=4096)])
 def test_bvh2_config_variants_bit_identical(cw, c0, c0_ref, kw):
     wl, lays = c0
     say(f"BVH2 variant {kw}")
@@ -316,7 +322,8 @@ def test_large_scene_uploads_128b_nodes(cw):
     about 320 MB of nodes and Woop rows) gets 128 B nodes by default, and its
     default configuration sorts launches of 2^20 rays or more by origin
     Morton code first; hits equal those of the 80 B layout traced in the
-    given order.  The BVH2 scene of the same mesh sorts by default too."""
+    given order.  The BVH2 scene of the same mesh sorts by default too and
+    traces on the persistent grid; its hits equal one ray per thread's."""
     say("building 5 M-triangle soup")
     mesh = cw.Mesh.cluster_soup(5_000_000, 1024, 0.005, 0.05, float(np.log(1e-3)), 0.5, seed=41)
     bvh2, cwb = cw.build_layouts(mesh)
@@ -378,8 +385,10 @@ def test_large_scene_uploads_128b_nodes(cw):
     assert_bitwise(a, b, "surfel trace, scene default vs explicit")
     assert (cw.hit_ids(a) >= 0).mean() > 0.05
     s2 = cw.Scene(bvh2)
-    assert s2.default_config().ray_order == 1
-    b2_given = s2.default_config()
+    d2 = s2.default_config()
+    assert d2.ray_order == 1 and d2.persistent == 2
+    # against one ray per thread in the given order: the reference's launch
+    b2_given = s2.default_config(persistent=0)
     b2_given.ray_order = 0
     for any_hit in (False, True):
         say(f"large BVH2 scene any={any_hit}")
@@ -400,6 +409,15 @@ def test_invalid_loop_configs_rejected(cw, c0):
         with pytest.raises(cw.CwbvhError) as e:
             sc.trace_numpy(wl.rays[:64], 0, cfg)
         assert e.value.status == 1, kw
+    # BVH2: the CWBVH-only fields stay 0; a persistent launch needs a refill budget
+    s2 = cw.Scene(lays["bvh2"])
+    for kw in (dict(loop_mode=1), dict(prefetch=1), dict(postpone_pct=10), dict(occupancy=7), dict(persistent=3),
+               dict(persistent=1, refill_budget=0), dict(persistent=2, refill_budget=4, work_slices=9),
+               dict(persistent=1, refill_budget=4, refill_idle=65)):
+        cfg = cw.default_trace_config(cw.KIND_BVH2, **kw)
+        with pytest.raises(cw.CwbvhError) as e:
+            s2.trace_numpy(wl.rays[:64], 0, cfg)
+        assert e.value.status == 1, kw
 
 
 def test_misaligned_buffers_rejected(cw, c0):
@@ -417,8 +435,10 @@ def test_misaligned_buffers_rejected(cw, c0):
         cw.Batch([(sc, dr.ptr + 16 + 4, dh.ptr, 64)])
 
 
-@pytest.mark.parametrize("kw", [None, dict(persistent=0), dict(loop_mode=2, prefetch=1)])
-def test_order_entries_out_of_range_stay_in_bounds(cw, c0, c0_ref, kw):
+@pytest.mark.parametrize("k,kw", [("cwbvh", None), ("bvh2", None), ("cwbvh", dict(persistent=0)),
+                                  ("cwbvh", dict(loop_mode=2, prefetch=1)), ("bvh2", dict(persistent=0)),
+                                  ("bvh2", dict(persistent=1, refill_idle=4, refill_budget=16, work_slices=8))])
+def test_order_entries_out_of_range_stay_in_bounds(cw, c0, c0_ref, k, kw):
     """An ordered trace whose order holds entries outside [0, n) traces ray i
     into slot i for those entries: with an otherwise identity order the hits
     equal the plain trace's, and no access leaves the buffers."""
@@ -426,14 +446,12 @@ def test_order_entries_out_of_range_stay_in_bounds(cw, c0, c0_ref, kw):
     n = len(wl.rays)
     order = np.arange(n, dtype=np.int32)
     order[[3, 100, 777, n - 1]] = [-5, n, 1 << 30, -(1 << 31)]
-    for k in ("cwbvh", "bvh2"):
-        if kw is not None and k == "bvh2":
-            continue
-        say(f"out-of-range order entries {k} {kw}")
-        sc = cw.Scene(lays[k])
-        cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw) if kw is not None else None
-        g = sc.trace_numpy(wl.rays, 0, cfg, order=order)
-        assert_bitwise(g, c0_ref[(k, False)], f"out-of-range order entries {k} {kw}")
+    say(f"out-of-range order entries {k} {kw}")
+    sc = cw.Scene(lays[k])
+    kind = cw.KIND_CWBVH if k == "cwbvh" else cw.KIND_BVH2
+    cfg = cw.default_trace_config(kind, **kw) if kw is not None else None
+    g = sc.trace_numpy(wl.rays, 0, cfg, order=order)
+    assert_bitwise(g, c0_ref[(k, False)], f"out-of-range order entries {k} {kw}")
 
 
 def test_sorted_launches_on_two_threads_and_streams(cw, c0, c0_ref):
