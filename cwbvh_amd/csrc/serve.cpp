@@ -171,6 +171,14 @@ struct cwbvh_server {
     void loop();
 };
 
+// lds_depth 0 in the server's trace configuration: each launch keeps the
+// deepest stack need of its trees in LDS, so the LDS-only kernels serve it
+// (10 entries when a tree's need is unknown)
+static int32_t launch_lds_depth(int32_t configured, int32_t max_need) {
+    if (configured > 0) return configured;
+    return max_need > 0 && max_need <= CWBVH_STACK_CAPACITY ? max_need : 10;
+}
+
 int32_t cwbvh_server::launch(Slot& sl, const std::vector<Request*>& batch) {
     if (batch[0]->kind == CWBVH_REQUEST_SURFELS) {
         // one fused generate-and-trace launch; per-surfel counts come back
@@ -191,6 +199,7 @@ int32_t cwbvh_server::launch(Slot& sl, const std::vector<Request*>& batch) {
         L.n = r->n;
         L.flags = cfg.flags & CWBVH_TRACE_ANY;
         L.cfg = cfg.trace;
+        L.cfg.lds_depth = launch_lds_depth(cfg.trace.lds_depth, L.scene->stack_need);
         L.cfg.prefetch = 0;   // the fused kernel has no node prefetch
         L.counters = sl.counters;
         L.stream = sl.stream;
@@ -248,6 +257,7 @@ int32_t cwbvh_server::launch(Slot& sl, const std::vector<Request*>& batch) {
         L.n = r->n;
         L.flags = cfg.flags;
         L.cfg = cfg.trace;
+        L.cfg.lds_depth = launch_lds_depth(cfg.trace.lds_depth, max_need);
         L.counters = sl.counters;
         L.stream = sl.stream;
         int32_t rc = launch_cwbvh(L);
@@ -260,9 +270,10 @@ int32_t cwbvh_server::launch(Slot& sl, const std::vector<Request*>& batch) {
         L.total_chunks = chunks;
         L.flags = cfg.flags;
         L.cfg = cfg.trace;
+        L.cfg.lds_depth = launch_lds_depth(cfg.trace.lds_depth, max_need);
         L.counters = sl.counters;
         L.stream = sl.stream;
-        L.lds_only = max_need > 0 && max_need <= cfg.trace.lds_depth && cfg.trace.lds_depth <= CWBVH_STACK_CAPACITY;
+        L.lds_only = max_need > 0 && max_need <= L.cfg.lds_depth && L.cfg.lds_depth <= CWBVH_STACK_CAPACITY;
         int32_t rc = launch_cwbvh_batch(L);
         if (rc) return rc;
     }
@@ -431,7 +442,9 @@ int32_t cwbvh_server_config_default(cwbvh_server_config* c) {
     c->max_batch_rays = (int64_t)16 << 20;
     c->serial = 0;
     c->flags = CWBVH_TRACE_CLOSEST;
-    return cwbvh_trace_config_default(CWBVH_KIND_CWBVH, &c->trace);
+    int32_t rc = cwbvh_trace_config_default(CWBVH_KIND_CWBVH, &c->trace);
+    c->trace.lds_depth = 0;  // per launch: the deepest stack need of its trees
+    return rc;
 }
 
 int32_t cwbvh_server_create(const cwbvh_scene* const* scenes, int32_t num_scenes, const cwbvh_server_config* cfg,
@@ -444,7 +457,10 @@ int32_t cwbvh_server_create(const cwbvh_scene* const* scenes, int32_t num_scenes
                      c.max_batch_requests >= 1 && c.max_batch_requests <= 4096 && c.max_batch_rays >= 1,
                  "invalid server configuration");
     CW_CHECK_ARG((c.flags & ~(uint32_t)(CWBVH_TRACE_ANY | CWBVH_TRACE_PRIM_ID)) == 0, "unsupported flags");
-    int32_t rc = validate_config(CWBVH_KIND_CWBVH, c.trace);
+    CW_CHECK_ARG(c.trace.lds_depth >= 0, "lds_depth must be >= 0 (0 = each launch's deepest stack need)");
+    cwbvh_trace_config vc = c.trace;
+    if (vc.lds_depth == 0) vc.lds_depth = CWBVH_STACK_CAPACITY;  // the deepest a launch can pick
+    int32_t rc = validate_config(CWBVH_KIND_CWBVH, vc);
     if (rc) return rc;
     for (int32_t i = 0; i < num_scenes; ++i)
         CW_CHECK_ARG(scenes[i] && scenes[i]->kind == CWBVH_KIND_CWBVH &&

@@ -495,7 +495,9 @@ typedef struct cwbvh_server_config {
     int64_t max_batch_rays;       /* rays per batch launch */
     int32_t serial;               /* 1: one request at a time with a plain per-scene launch */
     uint32_t flags;               /* CWBVH_TRACE_CLOSEST or CWBVH_TRACE_ANY (+ PRIM_ID) */
-    cwbvh_trace_config trace;     /* kernel configuration */
+    cwbvh_trace_config trace;     /* kernel configuration; trace.lds_depth 0 (the default) =
+                                     each launch keeps the deepest stack need of its trees in
+                                     LDS (10 when a tree's need is unknown) */
 } cwbvh_server_config;
 
 int32_t cwbvh_server_config_default(cwbvh_server_config* cfg);

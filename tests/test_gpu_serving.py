@@ -48,12 +48,16 @@ def oracle_hits(lay, rays):
     return oracle.trace_cwbvh(lay.nodes, lay.node_stride, lay.woop, rays)
 
 
-@pytest.mark.parametrize("serial", [0, 1])
-def test_requests_same_alone_and_in_mix(cw, serving_set, serial):
+# lds_depth 0 (the default): each launch's LDS depth is its trees' deepest
+# stack need (the LDS-only kernels); 3: the LDS + scratch stack
+@pytest.mark.parametrize("serial,lds_depth", [(0, 0), (1, 0), (0, 3), (1, 3)])
+def test_requests_same_alone_and_in_mix(cw, serving_set, serial, lds_depth):
     scenes, pools, lays = serving_set
     rng = np.random.default_rng(5)
     cfg = cw.default_server_config(serial=serial, max_inflight=3, max_batch_requests=8,
                                    memory_budget=64 << 20)
+    assert cfg.trace.lds_depth == 0
+    cfg.trace.lds_depth = lds_depth
     srv = cw.Server(scenes, cfg)
     reqs = []
     for k in range(40):
