@@ -483,6 +483,49 @@ def bench_c2(args, cw, W, dev) -> dict:
               "hit_rate": round(float(np.mean(np.concatenate([oracle.hit_ids(r) >= 0 for r in ref_all]))), 4)}
     log(f"parity C2: {mism} of {parity['rays']} sampled rays differ from the oracle")
 
+    # any hit (AO occlusion): the tuned configuration as an any-hit graph
+    # replay against 256 plain any-hit launches; its sampled hits against the
+    # oracle's any hit
+    def plain_any():
+        for sc, r, h, n in items:
+            sc.trace(r, h, n, cw.TRACE_ANY, None, None, st)
+
+    ms_plain_any = timed(plain_any, args.steps, args.warmup)
+    batch_any = cw.Batch(items, cw.TRACE_ANY, best.cfg, morton=bool(best.morton))
+    batch_any.capture(st)
+    ms_batch_any = timed(lambda: batch_any.replay(st), args.steps, args.warmup)
+    for _, dh in bufs:
+        dh.zero()
+    batch_any.replay(st)
+    st.synchronize()
+    mism_any = occl_any = unverified = 0
+    for (idx, srays), (_, dh), lay, it in zip(samples, bufs, lays, items):
+        got = dh.download((it[3], 4))[idx]
+        ref = oracle.trace_cwbvh(lay.nodes, lay.node_stride, lay.woop, srays, any_hit=True, threads=host_threads())
+        differ = (got.view(np.uint32) != ref.view(np.uint32)).any(axis=1)
+        mism_any += int(differ.sum())
+        gid, rid = oracle.hit_ids(got), oracle.hit_ids(ref)
+        occl_any += int(((gid >= 0) != (rid >= 0)).sum())
+        # a speculative schedule may report another triangle on the ray: it
+        # must be one the oracle's Woop test accepts with the same t, u, v
+        for i in np.nonzero(differ & (gid >= 0) & (rid >= 0))[0]:
+            ok, t, u, v = oracle.woop_test(lay.woop[gid[i]:gid[i] + 3], srays[i])
+            if not (ok and np.float32(t) == got[i, 0] and np.float32(u) == got[i, 2] and np.float32(v) == got[i, 3]):
+                unverified += 1
+    any_ok = occl_any == 0 and unverified == 0 and (mism_any == 0 or best.cfg.loop_mode == 2)
+    any_hit = {"batched_ms": round(ms_batch_any, 4), "batched_mrays_s": round(total / ms_batch_any / 1e3, 2),
+               "plain_ms": round(ms_plain_any, 4), "plain_mrays_s": round(total / ms_plain_any / 1e3, 2),
+               "speedup_vs_plain": round(ms_plain_any / ms_batch_any, 3),
+               "what": "the tuned configuration as an any-hit graph replay; plain = 256 any-hit launches",
+               "parity": {"rays": parity["rays"], "bitwise_mismatches": mism_any, "occlusion_mismatches": occl_any,
+                          "unverified_hits": unverified, "ok": any_ok,
+                          "what": "any-hit graph replay, the same samples, against the oracle's any hit: bitwise "
+                                  "(the speculative schedule: occlusion identical and every other reported hit "
+                                  "accepted by the oracle's Woop test)"}}
+    parity["ok"] = parity["ok"] and any_ok
+    log(f"C2 any hit: batched {ms_batch_any:.2f} ms, plain {ms_plain_any:.2f} ms, "
+        f"{mism_any} sampled rays differ from the oracle")
+
     # cache-side traffic of the batch (counting launches per scene, outside
     # the timed region)
     d_cnt = cw.DeviceBuffer(16)
@@ -509,6 +552,7 @@ def bench_c2(args, cw, W, dev) -> dict:
         "plain_roofline": roofline("cwbvh_trace<closest> x 256 plain launches", total, ms_plain, scene_bytes),
         "speedup_vs_plain": round(ms_plain / ms_batch, 3),
         "speedup_tuned_vs_untuned": round(ms_untuned / ms_batch, 4),
+        "any_hit": any_hit,
         "autotune": {"best": best.as_dict(), "seconds": round(tune_s, 1), "points": len(recs),
                      "fastest_5": [r.as_dict() for r in sorted(recs, key=lambda r: r.ms)[:5]],
                      "slowest_ms": round(max(r.ms for r in recs), 4)},
