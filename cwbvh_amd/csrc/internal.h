@@ -36,10 +36,11 @@ int32_t hip_fail(hipError_t e, const char* what);
     } while (0)
 
 // Per-stream work counters for persistent launches: uint32[kWorkWords] =
-// {fetch counters [0, kWorkSlices), exited waves, stack overflow flag, ...}.
-// The kernel resets the fetch counters and the exit count when its last wave
-// exits, so launches on one stream reuse the same counters (and a captured
-// graph replays cleanly).  The overflow flag stays set until read.
+// {fetch counters [0, kWorkSlices), exited block groups, stack overflow flag,
+// loop profile, drained-slice mask, exited blocks per group [0, kWorkSlices)}.
+// The kernel resets the fetch counters, the exit counts and the mask when its
+// last block exits, so launches on one stream reuse the same counters (and a
+// captured graph replays cleanly).  The overflow flag stays set until read.
 // Each counter sits on its own 256-byte line so that the slices' atomics are
 // served by different memory channels instead of serializing on one line.
 // Scenes larger than the MI355X Infinity Cache (256 MB) are traversed in the
@@ -54,7 +55,9 @@ constexpr int kWorkStride = 64;                       // uint32 per counter line
 constexpr int kWorkExited = kWorkSlices * kWorkStride;
 constexpr int kWorkOverflow = kWorkExited + kWorkStride;
 constexpr int kWorkProfile = kWorkOverflow + kWorkStride;   // uint64[6] loop-occupancy counters
-constexpr int kWorkWords = kWorkProfile + kWorkStride;
+constexpr int kWorkDrained = kWorkProfile + kWorkStride;    // bit s: slice s has no items left
+constexpr int kWorkGroupExit = kWorkDrained + kWorkStride;  // per home slice: blocks exited
+constexpr int kWorkWords = kWorkGroupExit + kWorkSlices * kWorkStride;
 int32_t stream_counters(hipStream_t stream, unsigned** out);
 // Frees the Morton workspace kept for `stream` on the current device.
 void release_stream_scratch(hipStream_t stream);

@@ -46,6 +46,80 @@ constexpr int kWorkStride = cw::kWorkStride;
 constexpr int kWorkExited = cw::kWorkExited;
 constexpr int kWorkOverflow = cw::kWorkOverflow;
 constexpr int kWorkProfile = cw::kWorkProfile;
+constexpr int kWorkDrained = cw::kWorkDrained;
+constexpr int kWorkGroupExit = cw::kWorkGroupExit;
+
+// Work distribution of the persistent grids.  The items (rays, or a batch's
+// chunk indices) are split into `slices` contiguous ranges with one claim
+// counter each.  Blocks are dispatched round-robin over the XCDs, so block b
+// takes slice b % slices (its home) first, then the others.
+//  * First claim without an atomic: wave i of the W_s waves whose home is
+//    slice s owns [i * amount, (i + 1) * amount) of it outright, and the
+//    counter hands out the slice from W_s * amount on.  Otherwise every wave
+//    of the grid claims at once when the launch starts, and the claims queue
+//    on eight counter lines.
+//  * Drained mask: the first wave to find a slice dry sets its bit in one
+//    word.  A wave that has run out of its home slice reads the word once and
+//    skips the drained slices, instead of probing every counter with an
+//    atomic at the end of the launch.
+//  * Exit: blocks count their exits per home slice; the last block of a home
+//    group counts on one word, and the last group resets everything, so no
+//    line takes an atomic from every block.
+__device__ __forceinline__ unsigned home_blocks(int s, int total_blocks, int slices) {
+    return s < total_blocks ? (unsigned)((total_blocks - 1 - s) / slices + 1) : 0u;
+}
+
+// Slice-relative base of the wave's next claim of `amount` items from slice
+// `sl` (>= the slice size when it has run dry).  `first`: the wave's first
+// claim, from its home slice, with index `first_index` among that slice's
+// waves.  The wave's first active lane does the memory work; the result is
+// wave-uniform (scalar registers).
+__device__ __forceinline__ unsigned slice_claim(unsigned* work, int sl, unsigned amount, bool first,
+                                                unsigned first_index, int total_blocks, int slices, int lane) {
+    // 64-bit sums, saturated: a value past 2^32 - 1 reads as a dry slice
+    if (first) return (unsigned)min((unsigned long long)first_index * amount, 0xFFFFFFFFull);
+    unsigned base = 0;
+    if (lane == __builtin_amdgcn_readfirstlane(lane)) base = atomicAdd(&work[sl * kWorkStride], amount);
+    base = __builtin_amdgcn_readfirstlane(base);
+    const unsigned long long static_items =
+        (unsigned long long)home_blocks(sl, total_blocks, slices) * (blockDim.x >> 6) * amount;
+    return (unsigned)min(static_items + base, 0xFFFFFFFFull);
+}
+
+// Marks slice `sl` drained and returns the drained mask as now known: one
+// load, and an atomic only when the slice is not marked yet.
+__device__ __forceinline__ unsigned mark_drained(unsigned* work, int sl, int lane) {
+    unsigned m = 0;
+    if (lane == __builtin_amdgcn_readfirstlane(lane)) {
+        m = __hip_atomic_load(&work[kWorkDrained], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (!((m >> sl) & 1u)) m = atomicOr(&work[kWorkDrained], 1u << sl);
+    }
+    return __builtin_amdgcn_readfirstlane(m) | (1u << sl);
+}
+
+// Exit of a persistent block (all threads call it).  Every wave has consumed
+// the result of its last claim before the barrier, so the claims are
+// complete when the exit is counted; the last block out resets the counters
+// for the next launch on the stream.
+__device__ __forceinline__ void block_exit(unsigned* work, int total_blocks, int slices) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int g = (int)(blockIdx.x % (unsigned)slices);
+        const unsigned prev = atomicAdd(&work[kWorkGroupExit + g * kWorkStride], 1u);
+        if (prev == home_blocks(g, total_blocks, slices) - 1u) {
+            const unsigned groups = (unsigned)min(slices, total_blocks);
+            const unsigned gprev = atomicAdd(&work[kWorkExited], 1u);
+            if (gprev == groups - 1u) {
+                for (int k = 0; k < kWorkSlices; ++k) {
+                    atomicExch(&work[k * kWorkStride], 0u);
+                    atomicExch(&work[kWorkGroupExit + k * kWorkStride], 0u);
+                }
+                atomicExch(&work[kWorkDrained], 0u);
+                atomicExch(&work[kWorkExited], 0u);
+            }
+        }
+    }
+}
 
 struct CwbvhParams {
     const float4* __restrict__ nodes;
@@ -228,6 +302,8 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
     bool exhausted = false;
     const int home = (int)(blockIdx.x % (unsigned)P.slices);
     int slice_k = 0;
+    const unsigned first_index =
+        (blockIdx.x / (unsigned)P.slices) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
     while (true) {
         const bool terminated = !busy;
@@ -258,18 +334,25 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                     // range, then from the others
                     bool found = false;
                     unsigned base = 0;
+                    // the wave's first claim: nothing claimed yet; slices found dry in this refill
+                    const bool first_claim = slice_k == 0 && pool_end == 0u;
+                    unsigned drained = 0;
                     while (slice_k < P.slices) {
                         const int sl = (home + slice_k) % P.slices;
+                        if (slice_k > 0 && ((drained >> sl) & 1u)) {
+                            ++slice_k;
+                            continue;
+                        }
                         const unsigned lo = (unsigned)(((unsigned long long)P.n * (unsigned)sl) / (unsigned)P.slices);
                         const unsigned hi = (unsigned)(((unsigned long long)P.n * (unsigned)(sl + 1)) / (unsigned)P.slices);
-                        unsigned claim = 0;
-                        if (lane == leader) claim = atomicAdd(&P.work[sl * kWorkStride], 1u);
-                        claim = __shfl(claim, leader);
+                        const unsigned claim = slice_claim(P.work, sl, 1u, first_claim && slice_k == 0, first_index, P.total_blocks,
+                                                           P.slices, lane);
                         if (claim < hi - lo) {
                             base = lo + claim;
                             found = true;
                             break;
                         }
+                        drained = mark_drained(P.work, sl, lane);
                         ++slice_k;
                     }
                     if (!found) {
@@ -289,19 +372,26 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                     pool_end = min(first + (unsigned)P.chunk, (unsigned)P.jobs[lo].n);
                 } else {
                     bool found = false;
+                    // the wave's first claim: nothing claimed yet; slices found dry in this refill
+                    const bool first_claim = slice_k == 0 && pool_end == 0u;
+                    unsigned drained = 0;
                     while (slice_k < P.slices) {
                         const int sl = (home + slice_k) % P.slices;
+                        if (slice_k > 0 && ((drained >> sl) & 1u)) {
+                            ++slice_k;
+                            continue;
+                        }
                         const unsigned lo = (unsigned)(((unsigned long long)P.n * (unsigned)sl) / (unsigned)P.slices);
                         const unsigned hi = (unsigned)(((unsigned long long)P.n * (unsigned)(sl + 1)) / (unsigned)P.slices);
-                        unsigned base = 0;
-                        if (lane == leader) base = atomicAdd(&P.work[sl * kWorkStride], (unsigned)P.chunk);
-                        base = __shfl(base, leader);
+                        const unsigned base = slice_claim(P.work, sl, (unsigned)P.chunk, first_claim && slice_k == 0, first_index,
+                                                          P.total_blocks, P.slices, lane);
                         if (base < hi - lo) {
                             pool_next = lo + base;
                             pool_end = lo + min(base + (unsigned)P.chunk, hi - lo);
                             found = true;
                             break;
                         }
+                        drained = mark_drained(P.work, sl, lane);
                         ++slice_k;
                     }
                     if (!found) {
@@ -597,19 +687,9 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
         atomicAdd(&pr[4], pr_tr);
         atomicAdd(&pr[5], pr_tru);
     }
-    // The last block out resets the fetch counters for the next launch on this
-    // stream (one-ray-per-thread launches claim nothing and skip this).  No
-    // fence is needed: every wave has consumed the result of its last claim
-    // before the barrier, so all claims are performed before the exit count.
+    // one-ray-per-thread launches claim nothing and skip the exit count
     if (!MULTI && !P.persistent) return;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned prev = atomicAdd(&P.work[kWorkExited], 1u);
-        if (prev == (unsigned)P.total_blocks - 1u) {
-            for (int k = 0; k < kWorkSlices; ++k) atomicExch(&P.work[k * kWorkStride], 0u);
-            atomicExch(&P.work[kWorkExited], 0u);
-        }
-    }
+    block_exit(P.work, P.total_blocks, P.slices);
 }
 
 struct Bvh2Params {
@@ -777,6 +857,8 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace_persistent(Bvh2Params P)
     bool exhausted = false;
     const int home = (int)(blockIdx.x % (unsigned)P.slices);
     int slice_k = 0;
+    const unsigned first_index =
+        (blockIdx.x / (unsigned)P.slices) * (blockDim.x >> 6) + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
 
     while (true) {
         const bool idle = !busy;
@@ -788,19 +870,26 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace_persistent(Bvh2Params P)
                 if (exhausted) break;
                 const int leader = __ffsll((long long)need) - 1;
                 bool found = false;
+                // the wave's first claim: nothing claimed yet; slices found dry in this refill
+                const bool first_claim = slice_k == 0 && pool_end == 0u;
+                unsigned drained = 0;
                 while (slice_k < P.slices) {
                     const int sl = (home + slice_k) % P.slices;
+                    if (slice_k > 0 && ((drained >> sl) & 1u)) {
+                        ++slice_k;
+                        continue;
+                    }
                     const unsigned lo = (unsigned)(((unsigned long long)P.n * (unsigned)sl) / (unsigned)P.slices);
                     const unsigned hi = (unsigned)(((unsigned long long)P.n * (unsigned)(sl + 1)) / (unsigned)P.slices);
-                    unsigned base = 0;
-                    if (lane == leader) base = atomicAdd(&P.work[sl * kWorkStride], (unsigned)P.chunk);
-                    base = __shfl(base, leader);
+                    const unsigned base = slice_claim(P.work, sl, (unsigned)P.chunk, first_claim && slice_k == 0, first_index,
+                                                      P.total_blocks, P.slices, lane);
                     if (base < hi - lo) {
                         pool_next = lo + base;
                         pool_end = lo + min(base + (unsigned)P.chunk, hi - lo);
                         found = true;
                         break;
                     }
+                    drained = mark_drained(P.work, sl, lane);
                     ++slice_k;
                 }
                 if (!found) {
@@ -922,16 +1011,7 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace_persistent(Bvh2Params P)
         atomicAdd(&P.stats[0], st_nodes);
         atomicAdd(&P.stats[1], st_tris);
     }
-    // every wave has consumed the result of its last claim before this
-    // barrier, so the claims are complete when the exit is counted
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned prev = atomicAdd(&P.work[kWorkExited], 1u);
-        if (prev == (unsigned)P.total_blocks - 1u) {
-            for (int k = 0; k < kWorkSlices; ++k) atomicExch(&P.work[k * kWorkStride], 0u);
-            atomicExch(&P.work[kWorkExited], 0u);
-        }
-    }
+    block_exit(P.work, P.total_blocks, P.slices);
 }
 
 __global__ void copy_u4_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, long long n) {
