@@ -1094,7 +1094,10 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    # the first ~20 C1 launches after idle run slower while the GPU clocks ramp
+    # up (2.97 ms falling to 2.14 ms over about 45 ms of launches); 30 warm-up
+    # steps (about 125 ms of C1 launches) keep the ramp out of the timed steps
+    ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--config", default="all", choices=["all", "c0", "c1", "c2", "c3", "c4", "files", "ao", "profile"])
     ap.add_argument("--configs", default=",".join(CHILD_ORDER), help="--config all: which configurations to run")
     ap.add_argument("--deadline-s", type=float, default=540.0, help="--config all: wall-clock budget of the run")
