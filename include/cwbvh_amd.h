@@ -347,7 +347,7 @@ typedef struct cwbvh_trace_config {
     int32_t prefetch;        /* 0 or 1: issue the next node's loads before the triangle loop */
     int32_t work_slices;     /* 1..8 contiguous ray ranges, one drained first per XCD (persistent grids) */
     int32_t persistent;      /* 1 = persistent grid with work fetching, 0 = one ray per thread,
-                                2 = persistent from 2^19 rays up.  CWBVH default 2.  BVH2 default 0
+                                2 = persistent from 786,432 (3 x 2^18) rays up.  CWBVH default 2.  BVH2 default 0
                                 (the reference's kernel); its persistent grid hands rays to idle lanes
                                 between while-while rounds and uses grid_blocks, fetch_chunk,
                                 refill_idle (>= 0), refill_budget (>= 1) and work_slices.  BVH2
