@@ -487,9 +487,10 @@ int32_t cwbvh_trace_config_for_scene(const cwbvh_scene* scene, cwbvh_trace_confi
             c->prefetch = scene->bounds_known ? 1 : 0;
         } else {
             // BVH2 on a tree that streams from HBM: the persistent grid keeps
-            // lanes busy while the slowest rays of a wave finish (C3 sorted,
-            // 67 M rays: 176 ms against 197 one ray per thread; on the
-            // cache-resident C1 it gains 4.5% any hit, loses 1% closest)
+            // lanes busy while the slowest rays of a wave finish (20 M-triangle
+            // soup, 67 M sorted rays: 176 ms against 197 one ray per thread;
+            // on a cache-resident 1 M-triangle scene it gains 4.5% any hit and
+            // loses 1% closest hit)
             c->persistent = 2;
             c->refill_idle = 2;
             c->refill_budget = 8;
