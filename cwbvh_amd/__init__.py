@@ -255,6 +255,7 @@ _SIGS = {
     "cwbvh_trace_ordered": (_I32, [_P, _P, _P, _P, _I64, _U32, _P, _P, _P]),
     "cwbvh_stack_overflowed": (_I32, [_P, C.POINTER(_I32)]),
     "cwbvh_loop_profile": (_I32, [_P, _P]),
+    "cwbvh_rcp_check": (_I32, [_P, _P]),
     "cwbvh_trace_config_for_scene": (_I32, [_P, _P]),
     "cwbvh_scene_bounds": (_I32, [_P, _P, _P]),
     "cwbvh_morton_order": (_I32, [_P, _I64, _P, _P, _P, _P]),
@@ -1071,6 +1072,21 @@ def stack_overflowed(stream: Stream | None = None) -> bool:
     f = C.c_int32()
     check(lib.cwbvh_stack_overflowed(stream.handle if stream else None, C.byref(f)))
     return bool(f.value)
+
+
+def rcp_check(stream: Stream | None = None) -> dict:
+    """Runs cwbvh_rcp_check over all 2^32 float patterns and returns its
+    counts: in-range mismatches of the refined reciprocal (0 when the ray
+    setup's identity holds on this device), in-range patterns, and
+    mismatches outside the range."""
+    d = DeviceBuffer(24)
+    check(lib.cwbvh_rcp_check(d.ptr, stream.handle if stream else None))
+    if stream is not None:
+        stream.synchronize()
+    else:
+        synchronize()
+    c = d.download((3,), np.uint64)
+    return {"in_range_mismatches": int(c[0]), "in_range": int(c[1]), "outside_mismatches": int(c[2])}
 
 
 def loop_profile(stream: Stream | None = None) -> dict:

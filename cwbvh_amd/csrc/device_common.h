@@ -32,10 +32,40 @@ __device__ __forceinline__ float imax2(float a, float b) { return i2f(max(f2i(a)
 __device__ __forceinline__ float imin3(float a, float b, float c) { return i2f(min(min(f2i(a), f2i(b)), f2i(c))); }
 __device__ __forceinline__ float imax3(float a, float b, float c) { return i2f(max(max(f2i(a), f2i(b)), f2i(c))); }
 
-// 1 / d with |d| clamped to 2^-80 (reference src/TraversalKernelCWBVH.cu:78-80).
-__device__ __forceinline__ float safe_rcp(float d) {
+// |d| clamped to 2^-80 (reference src/TraversalKernelCWBVH.cu:78-80); a NaN
+// becomes +-2^-80 as well.
+__device__ __forceinline__ float clamp_dir(float d) {
     const float ooeps = 0x1p-80f;
-    return 1.0f / (fabsf(d) > ooeps ? d : copysignf(ooeps, d));
+    return fabsf(d) > ooeps ? d : copysignf(ooeps, d);
+}
+
+// 1 / x as v_rcp_f32 refined by one fma Newton step.  For every binary32 x
+// with |x| in [2^-125, 2^125] this equals the IEEE quotient 1.0f / x (all
+// such values compared on gfx950); outside that range it may not.
+__device__ __forceinline__ float rcp_refined(float x) {
+    const float r = __builtin_amdgcn_rcpf(x);
+    return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
+}
+
+// 1 / d per component with |d| clamped to 2^-80, i.e. the IEEE quotients of
+// the reference's ray setup (src/TraversalKernelCWBVH.cu:78-80, and
+// src/TraversalKernelBVH2.cu:52-55).  Clamped components are at least
+// 2^-80 in magnitude, so the refined reciprocal (3 instructions) returns the
+// quotient's bits unless a component exceeds 2^125 or is infinite; the lanes
+// calling together take the IEEE division (11 instructions each) when any of
+// them holds such a component.
+__device__ __forceinline__ void safe_rcp3(const float3 d, float& ix, float& iy, float& iz) {
+    const float cx = clamp_dir(d.x), cy = clamp_dir(d.y), cz = clamp_dir(d.z);
+    const bool in_range = fabsf(cx) <= 0x1p125f && fabsf(cy) <= 0x1p125f && fabsf(cz) <= 0x1p125f;
+    if (__all(in_range)) {
+        ix = rcp_refined(cx);
+        iy = rcp_refined(cy);
+        iz = rcp_refined(cz);
+    } else {
+        ix = 1.0f / cx;
+        iy = 1.0f / cy;
+        iz = 1.0f / cz;
+    }
 }
 
 // Woop ray/triangle test (reference src/TraversalKernelCWBVH.cu:318-347).

@@ -68,8 +68,18 @@ constexpr int kWorkGroupExit = cw::kWorkGroupExit;
 //  * Exit: blocks count their exits per home slice; the last block of a home
 //    group counts on one word, and the last group resets everything, so no
 //    line takes an atomic from every block.
-__device__ __forceinline__ unsigned home_blocks(int s, int total_blocks, int slices) {
-    return s < total_blocks ? (unsigned)((total_blocks - 1 - s) / slices + 1) : 0u;
+//  * Splits: the host passes n = item_q * slices + item_r and the grid's
+//    blocks = block_q * slices + block_r, so the claims divide nothing (a
+//    64-bit division by a run-time value costs about 40 VALU instructions).
+// Blocks whose home is slice s (blocks b with b mod slices = s).
+__device__ __forceinline__ unsigned home_blocks(int s, unsigned block_q, unsigned block_r) {
+    return block_q + ((unsigned)s < block_r ? 1u : 0u);
+}
+
+// First item of slice s: the first item_r slices hold item_q + 1 items, the
+// others item_q.
+__device__ __forceinline__ unsigned slice_begin(int s, unsigned item_q, unsigned item_r) {
+    return (unsigned)s * item_q + min((unsigned)s, item_r);
 }
 
 // Slice-relative base of the wave's next claim of `amount` items from slice
@@ -78,14 +88,14 @@ __device__ __forceinline__ unsigned home_blocks(int s, int total_blocks, int sli
 // waves.  The wave's first active lane does the memory work; the result is
 // wave-uniform (scalar registers).
 __device__ __forceinline__ unsigned slice_claim(unsigned* work, int sl, unsigned amount, bool first,
-                                                unsigned first_index, int total_blocks, int slices, int lane) {
+                                                unsigned first_index, unsigned block_q, unsigned block_r, int lane) {
     // 64-bit sums, saturated: a value past 2^32 - 1 reads as a dry slice
     if (first) return (unsigned)min((unsigned long long)first_index * amount, 0xFFFFFFFFull);
     unsigned base = 0;
     if (lane == __builtin_amdgcn_readfirstlane(lane)) base = atomicAdd(&work[sl * kWorkStride], amount);
     base = __builtin_amdgcn_readfirstlane(base);
     const unsigned long long static_items =
-        (unsigned long long)home_blocks(sl, total_blocks, slices) * (blockDim.x >> 6) * amount;
+        (unsigned long long)home_blocks(sl, block_q, block_r) * (blockDim.x >> 6) * amount;
     return (unsigned)min(static_items + base, 0xFFFFFFFFull);
 }
 
@@ -104,12 +114,13 @@ __device__ __forceinline__ unsigned mark_drained(unsigned* work, int sl, int lan
 // the result of its last claim before the barrier, so the claims are
 // complete when the exit is counted; the last block out resets the counters
 // for the next launch on the stream.
-__device__ __forceinline__ void block_exit(unsigned* work, int total_blocks, int slices) {
+__device__ __forceinline__ void block_exit(unsigned* work, int total_blocks, int slices, unsigned block_q,
+                                           unsigned block_r) {
     __syncthreads();
     if (threadIdx.x == 0) {
         const int g = (int)(blockIdx.x % (unsigned)slices);
         const unsigned prev = atomicAdd(&work[kWorkGroupExit + g * kWorkStride], 1u);
-        if (prev == home_blocks(g, total_blocks, slices) - 1u) {
+        if (prev == home_blocks(g, block_q, block_r) - 1u) {
             const unsigned groups = (unsigned)min(slices, total_blocks);
             const unsigned gprev = atomicAdd(&work[kWorkExited], 1u);
             if (gprev == groups - 1u) {
@@ -147,6 +158,8 @@ struct CwbvhParams {
     int prim_id;
     int total_waves;
     int total_blocks;
+    unsigned item_q, item_r;        // n = item_q * slices + item_r
+    unsigned block_q, block_r;      // total_blocks = block_q * slices + block_r
     int spec_limit;                 // single scene: speculation_limit of its tree
     int lds_only;                   // host side: the tree's stack need fits lds_depth (LDS-only kernels)
     // SRC = kRaysFromSurfels: generate ray r from surfel r / rays_per_texel
@@ -338,9 +351,9 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                     bool found = false;
                     unsigned base = 0;
                     while (slice_k < P.slices) {
-                        const int sl = (home + slice_k) % P.slices;
-                        const unsigned lo = (unsigned)(((unsigned long long)P.n * (unsigned)sl) / (unsigned)P.slices);
-                        const unsigned hi = (unsigned)(((unsigned long long)P.n * (unsigned)(sl + 1)) / (unsigned)P.slices);
+                        const int sl = home + slice_k - (home + slice_k >= P.slices ? P.slices : 0);
+                        const unsigned lo = slice_begin(sl, P.item_q, P.item_r);
+                        const unsigned hi = slice_begin(sl + 1, P.item_q, P.item_r);
                         unsigned claim = 0;
                         if (lane == leader) claim = atomicAdd(&P.work[sl * kWorkStride], 1u);
                         claim = __shfl(claim, leader);
@@ -372,15 +385,15 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                     const bool first_claim = slice_k == 0 && pool_end == 0u;
                     unsigned drained = 0;
                     while (slice_k < P.slices) {
-                        const int sl = (home + slice_k) % P.slices;
+                        const int sl = home + slice_k - (home + slice_k >= P.slices ? P.slices : 0);
                         if (slice_k > 0 && ((drained >> sl) & 1u)) {
                             ++slice_k;
                             continue;
                         }
-                        const unsigned lo = (unsigned)(((unsigned long long)P.n * (unsigned)sl) / (unsigned)P.slices);
-                        const unsigned hi = (unsigned)(((unsigned long long)P.n * (unsigned)(sl + 1)) / (unsigned)P.slices);
+                        const unsigned lo = slice_begin(sl, P.item_q, P.item_r);
+                        const unsigned hi = slice_begin(sl + 1, P.item_q, P.item_r);
                         const unsigned base = slice_claim(P.work, sl, (unsigned)P.chunk, first_claim && slice_k == 0, first_index,
-                                                          P.total_blocks, P.slices, lane);
+                                                          P.block_q, P.block_r, lane);
                         if (base < hi - lo) {
                             pool_next = lo + base;
                             pool_end = lo + min(base + (unsigned)P.chunk, hi - lo);
@@ -451,9 +464,7 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
                 tmin = ro.w;
                 tmax = rd.w;
             }
-            idx = safe_rcp(d.x);
-            idy = safe_rcp(d.y);
-            idz = safe_rcp(d.z);
+            safe_rcp3(d, idx, idy, idz);
             const unsigned oct = ((d.x < 0.f ? 1u : 0u) << 2) | ((d.y < 0.f ? 1u : 0u) << 1) | (d.z < 0.f ? 1u : 0u);
             octinv = 7u - oct;
             octinv4 = octinv * 0x01010101u;
@@ -695,7 +706,7 @@ __global__ __launch_bounds__(kMaxBlock) CWBVH_TRACE_ATTR void cwbvh_trace(CwbvhP
             }
         }
     } else {
-        block_exit(P.work, P.total_blocks, P.slices);
+        block_exit(P.work, P.total_blocks, P.slices, P.block_q, P.block_r);
     }
 }
 
@@ -718,6 +729,8 @@ struct Bvh2Params {
     int idle_lanes;
     int idle_budget;
     int total_blocks;
+    unsigned item_q, item_r;   // n = item_q * slices + item_r
+    unsigned block_q, block_r; // total_blocks = block_q * slices + block_r
 };
 
 // Slab test of one BVH2 child box (reference src/TraversalKernelBVH2.cu:14-24).
@@ -750,7 +763,8 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace(Bvh2Params P) {
     const float3 o = make_float3(ro.x, ro.y, ro.z), d = make_float3(rd.x, rd.y, rd.z);
     const float tmin = ro.w;
     float hitT = rd.w;
-    const float idx = safe_rcp(d.x), idy = safe_rcp(d.y), idz = safe_rcp(d.z);
+    float idx, idy, idz;
+    safe_rcp3(d, idx, idy, idz);
     const float oodx = o.x * idx, oody = o.y * idy, oodz = o.z * idz;
     unsigned long long st_nodes = 0, st_tris = 0;
 
@@ -875,21 +889,20 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace_persistent(Bvh2Params P)
         while (need != 0ull) {
             if (pool_next >= pool_end) {
                 if (exhausted) break;
-                const int leader = __ffsll((long long)need) - 1;
                 bool found = false;
                 // the wave's first claim: nothing claimed yet; slices found dry in this refill
                 const bool first_claim = slice_k == 0 && pool_end == 0u;
                 unsigned drained = 0;
                 while (slice_k < P.slices) {
-                    const int sl = (home + slice_k) % P.slices;
+                    const int sl = home + slice_k - (home + slice_k >= P.slices ? P.slices : 0);
                     if (slice_k > 0 && ((drained >> sl) & 1u)) {
                         ++slice_k;
                         continue;
                     }
-                    const unsigned lo = (unsigned)(((unsigned long long)P.n * (unsigned)sl) / (unsigned)P.slices);
-                    const unsigned hi = (unsigned)(((unsigned long long)P.n * (unsigned)(sl + 1)) / (unsigned)P.slices);
+                    const unsigned lo = slice_begin(sl, P.item_q, P.item_r);
+                    const unsigned hi = slice_begin(sl + 1, P.item_q, P.item_r);
                     const unsigned base = slice_claim(P.work, sl, (unsigned)P.chunk, first_claim && slice_k == 0, first_index,
-                                                      P.total_blocks, P.slices, lane);
+                                                      P.block_q, P.block_r, lane);
                     if (base < hi - lo) {
                         pool_next = lo + base;
                         pool_end = lo + min(base + (unsigned)P.chunk, hi - lo);
@@ -925,9 +938,7 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace_persistent(Bvh2Params P)
             d = make_float3(rd.x, rd.y, rd.z);
             tmin = ro.w;
             hitT = rd.w;
-            idx = safe_rcp(d.x);
-            idy = safe_rcp(d.y);
-            idz = safe_rcp(d.z);
+            safe_rcp3(d, idx, idy, idz);
             oodx = o.x * idx;
             oody = o.y * idy;
             oodz = o.z * idz;
@@ -1018,7 +1029,7 @@ __global__ __launch_bounds__(kMaxBlock) void bvh2_trace_persistent(Bvh2Params P)
         atomicAdd(&P.stats[0], st_nodes);
         atomicAdd(&P.stats[1], st_tris);
     }
-    block_exit(P.work, P.total_blocks, P.slices);
+    block_exit(P.work, P.total_blocks, P.slices, P.block_q, P.block_r);
 }
 
 __global__ void copy_u4_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, long long n) {
@@ -1041,6 +1052,26 @@ __global__ void copy_u4_kernel(uint4* __restrict__ dst, const uint4* __restrict_
 constexpr int kValuProbeFma = 0;
 constexpr int kValuProbeInt = 1;
 constexpr int kValuProbeSlab = 2;
+
+// Every binary32 pattern x, by magnitude: counts[0] += |x| in [2^-125, 2^125]
+// and rcp_refined(x) differs from 1.0f / x; counts[1] += |x| in that range;
+// counts[2] += |x| outside it and the two differ.
+__global__ __launch_bounds__(256) void rcp_check_kernel(unsigned long long* __restrict__ counts) {
+    const unsigned long long total = 1ull << 32;
+    const unsigned long long stride = (unsigned long long)gridDim.x * blockDim.x;
+    unsigned long long bad_in = 0, n_in = 0, bad_out = 0;
+    for (unsigned long long i = (unsigned long long)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const float x = u2f((unsigned)i);
+        const bool in_range = fabsf(x) >= 0x1p-125f && fabsf(x) <= 0x1p125f;
+        const bool differ = f2u(rcp_refined(x)) != f2u(1.0f / x);
+        bad_in += (in_range && differ) ? 1 : 0;
+        n_in += in_range ? 1 : 0;
+        bad_out += (!in_range && differ) ? 1 : 0;
+    }
+    atomicAdd(&counts[0], bad_in);
+    atomicAdd(&counts[1], n_in);
+    atomicAdd(&counts[2], bad_out);
+}
 
 template <int MODE>
 __global__ __launch_bounds__(256) void valu_probe_kernel(float* __restrict__ out, long long iters, float a, float b) {
@@ -1104,6 +1135,16 @@ int resident_blocks(K kernel, int block, size_t lds) {
     return nb;
 }
 
+// Items (rays or chunks) and grid blocks split over the work slices.
+template <class Params>
+void set_splits(Params& p) {
+    const unsigned sl = (unsigned)std::max(1, p.slices);
+    p.item_q = (unsigned)p.n / sl;
+    p.item_r = (unsigned)p.n % sl;
+    p.block_q = (unsigned)p.total_blocks / sl;
+    p.block_r = (unsigned)p.total_blocks % sl;
+}
+
 template <bool ANY, bool STATS, bool MULTI, bool PREFETCH, int LOOP, int SRC = kRaysFromBuffer, bool W7 = false,
           bool LDSO = false>
 int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_t stream) {
@@ -1119,6 +1160,7 @@ int32_t launch_cwbvh_t(CwbvhParams& p, const cwbvh_trace_config& cfg, hipStream_
     }
     p.total_waves = (int)(grid * (block / 64));
     p.total_blocks = (int)grid;
+    set_splits(p);
     if (p.chunk <= 0) {
         // automatic: about 32 claims per wave over the launch, 128..512 rays each
         long long c = (long long)p.n / ((long long)p.total_waves * 32);
@@ -1148,6 +1190,7 @@ int32_t launch_bvh2_t(Bvh2Params& p, const cwbvh_trace_config& cfg, bool persist
     p.idle_lanes = cfg.refill_idle;
     p.idle_budget = cfg.refill_budget;
     p.chunk = cfg.fetch_chunk;
+    set_splits(p);
     if (p.chunk <= 0) {
         // automatic: about 32 claims per wave over the launch, 128..512 rays each
         const long long waves = grid * (block / 64);
@@ -1438,6 +1481,15 @@ extern "C" int32_t cwbvh_copy_u4(void* dst, const void* src, int64_t n_u4, void*
     if (grid > 65536) grid = 65536;
     hipLaunchKernelGGL(copy_u4_kernel, dim3((unsigned)grid), dim3(block), 0, (hipStream_t)stream, (uint4*)dst,
                        (const uint4*)src, (long long)n_u4);
+    CW_HIP(hipGetLastError());
+    return CWBVH_OK;
+}
+
+extern "C" int32_t cwbvh_rcp_check(uint64_t* d_counts, void* stream) {
+    CW_CHECK_ARG(d_counts != nullptr, "d_counts is null");
+    hipStream_t st = (hipStream_t)stream;
+    CW_HIP(hipMemsetAsync(d_counts, 0, 3 * sizeof(uint64_t), st));
+    hipLaunchKernelGGL(rcp_check_kernel, dim3(8192), dim3(256), 0, st, (unsigned long long*)d_counts);
     CW_HIP(hipGetLastError());
     return CWBVH_OK;
 }

@@ -289,6 +289,13 @@ int32_t cwbvh_event_destroy(void* event);
 int32_t cwbvh_event_record(void* event, void* stream);
 int32_t cwbvh_event_synchronize(void* event);
 int32_t cwbvh_event_elapsed_ms(void* start, void* stop, float* ms);
+/* Self-check of the ray setup's reciprocal (no counterpart in the reference):
+ * over all 2^32 binary32 patterns x, d_counts[0] = patterns with |x| in
+ * [2^-125, 2^125] whose v_rcp_f32 + one Newton step differs from the IEEE
+ * quotient 1.0f / x (the kernels rely on 0), d_counts[1] = patterns in that
+ * range (4,194,304,002), d_counts[2] = differing patterns outside it.
+ * d_counts: 3 device uint64, cleared by the call; asynchronous on `stream`. */
+int32_t cwbvh_rcp_check(uint64_t* d_counts, void* stream);
 /* Calibration kernel: dst[i] = src[i] for n uint4 words (HBM copy roofline). */
 int32_t cwbvh_copy_u4(void* dst, const void* src, int64_t n_u4, void* stream);
 /* Calibration kernel for the VALU issue ceiling: `blocks` blocks of 256

@@ -566,12 +566,14 @@ def test_edge_cases(cw, orc):
 def _pathological_rays():
     """Rays from above a heightfield, 64 per kind: NaN origin, NaN direction,
     zero direction, tmin > tmax, negative tmin, tmin = -0, huge origins,
-    NaN tmax, NaN tmin, a direction of length 1e-30."""
+    NaN tmax, NaN tmin, a direction of length 1e-30; then direction z
+    components of -1e38, exactly -2^125 and the next float below it (the ray
+    setup's reciprocal takes the IEEE division beyond 2^125), and -infinity."""
     rng = np.random.default_rng(12)
-    r = np.zeros((640, 8), np.float32)
-    r[:, :3] = rng.uniform(-0.2, 1.2, size=(640, 3))
+    r = np.zeros((768, 8), np.float32)
+    r[:, :3] = rng.uniform(-0.2, 1.2, size=(768, 3))
     r[:, 2] = 1.0
-    d = rng.normal(size=(640, 3))
+    d = rng.normal(size=(768, 3))
     d[:, 2] = -np.abs(d[:, 2]) - 0.5
     r[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
     r[:, 7] = np.inf
@@ -586,27 +588,46 @@ def _pathological_rays():
     r[448:512, 7] = nan
     r[512:576, 3] = nan
     r[576:640, 4:7] = [0.0, 0.0, -1e-30]
+    two125 = np.float32(2.0 ** 125)
+    r[640:672, 6] = -1e38
+    r[672:688, 6] = -two125
+    r[688:704, 6] = -np.nextafter(two125, np.float32(np.inf))
+    r[704:768, 6] = -np.inf
     return r
 
 
-@pytest.mark.parametrize("kw", [None, dict(loop_mode=0), dict(loop_mode=2), dict(persistent=0)])
-def test_pathological_rays(cw, orc, kw):
-    """NaN, zero-length and inverted-interval rays terminate and return what
-    the oracle returns, bit for bit (NaN rays miss)."""
+@pytest.mark.parametrize("kind,kw", [("both", None), ("cwbvh", dict(loop_mode=0)), ("cwbvh", dict(loop_mode=2)),
+                                     ("cwbvh", dict(persistent=0)), ("cwbvh", dict(persistent=1, refill_budget=1)),
+                                     ("cwbvh", dict(persistent=1, occupancy=7)),
+                                     ("bvh2", dict(persistent=1, refill_idle=2, refill_budget=8, work_slices=8))])
+def test_pathological_rays(cw, orc, kind, kw):
+    """NaN, zero-length, inverted-interval and huge-direction rays terminate
+    and return what the oracle returns, bit for bit (NaN rays miss)."""
     mesh = cw.Mesh.heightfield(32, 16, 8, 0.1, seed=2)
     bvh2, cwb = cw.build_layouts(mesh)
     rays = _pathological_rays()
     for lay in (cwb, bvh2):
-        if kw is not None and lay.kind != cw.KIND_CWBVH:
+        if kind != "both" and lay.kind != (cw.KIND_CWBVH if kind == "cwbvh" else cw.KIND_BVH2):
             continue
         sc = cw.Scene(lay)
-        cfg = cw.default_trace_config(cw.KIND_CWBVH, **kw) if kw is not None else None
+        cfg = cw.default_trace_config(lay.kind, **kw) if kw is not None else None
         for any_hit in (False, True):
             say(f"pathological rays kind={lay.kind} {kw} any={any_hit}")
             g = sc.trace_numpy(rays, int(any_hit), cfg)
             ref = oracle_trace(orc, lay, rays, any_hit)
             assert_bitwise(g, ref, f"pathological kind={lay.kind} any={any_hit}")
             assert np.all(orc.hit_ids(ref)[:128] == -1)
+
+
+def test_refined_reciprocal_equals_division(cw):
+    """The ray setup takes 1/d as v_rcp_f32 plus one Newton step when every
+    clamped component lies in [2^-125, 2^125]; on this device that equals the
+    IEEE quotient for every such float (all 2^32 patterns checked)."""
+    say("reciprocal check over all float patterns")
+    c = cw.rcp_check()
+    assert c["in_range"] == 4_194_304_002
+    assert c["in_range_mismatches"] == 0
+    assert c["outside_mismatches"] > 0  # outside the range the two do differ, and the check sees it
 
 
 def test_single_triangle_and_empty_scene(cw, orc):
