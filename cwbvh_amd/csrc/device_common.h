@@ -74,7 +74,7 @@ __device__ __forceinline__ void safe_rcp3(const float3 d, float& ix, float& iy, 
 // Accept: tmin < t, 0 <= u <= 1, 0 <= v, u + v <= 1, and
 //   ANY:     t < tmax
 //   closest: t < tmax, or t == tmax and tri_addr < hit_addr (deterministic ties)
-// 1/Dz is the IEEE quotient: computed as rcp_refined (4 instructions against
+// 1/Dz is the IEEE quotient: computed as rcp_refined (3 instructions against
 // the division's 11) when every calling lane's |Dz| lies in [2^-125, 2^125],
 // where the two agree, and by the division otherwise.
 template <bool ANY>
