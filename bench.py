@@ -392,17 +392,9 @@ def bench_single(args, cw, W, dev, name: str) -> dict:
 # ------------------------------------------------------------------- C2
 
 
-def bench_c2(args, cw, W, dev) -> dict:
-    """C2: many scenes, uneven AO batches.  Plain = one cwbvh_trace launch per
-    scene (default configuration) back to back on one stream; batched = one
-    autotuned persistent multi-scene launch replayed from a hipGraph (the
-    Morton pre-pass, when the tuner picks it, is inside the timed region)."""
+def build_c2(args, cw, W) -> list:
+    """C2's scenes and AO rays: [(plan entry, triangles, CWBVH layout, rays)]."""
     from concurrent.futures import ThreadPoolExecutor
-
-    import oracle
-
-    plan = W.c2_plan(num_scenes=args.c2_scenes)
-    t0 = time.perf_counter()
 
     def make(entry):
         mesh, rays, _ = W.c2_scene(entry)
@@ -410,7 +402,18 @@ def bench_c2(args, cw, W, dev) -> dict:
         return entry, mesh.num_triangles, lay, rays
 
     with ThreadPoolExecutor(max_workers=max(1, host_threads() // 2)) as ex:
-        built = list(ex.map(make, plan))
+        return list(ex.map(make, W.c2_plan(num_scenes=args.c2_scenes)))
+
+
+def bench_c2(args, cw, W, dev) -> dict:
+    """C2: many scenes, uneven AO batches.  Plain = one cwbvh_trace launch per
+    scene (default configuration) back to back on one stream; batched = one
+    autotuned persistent multi-scene launch replayed from a hipGraph (the
+    Morton pre-pass, when the tuner picks it, is inside the timed region)."""
+    import oracle
+
+    t0 = time.perf_counter()
+    built = build_c2(args, cw, W)
     build_s = time.perf_counter() - t0
     tris = sum(b[1] for b in built)
     log(f"C2: {len(built)} scenes, {tris} triangles, {sum(len(b[3]) for b in built)} rays, build {build_s:.1f} s")
@@ -538,6 +541,8 @@ def bench_c2(args, cw, W, dev) -> dict:
         nodes_c += int(c[0])
         tris_c += int(c[1])
     bytes_c = cache_bytes(1, total, nodes_c, tris_c)
+    # measured L2 <-> fabric bytes of the library-default batch launch over the same rays
+    rec = profile_record("cwbvh_batch_c2_traffic", rays=total, scenes=len(items), batch="library default")
     out = base_line(METRIC + " [C2 batched]", value, "Mrays/s", args, ms_batch,
                     {"workload": f"C2: {len(items)} seeded scenes (heightfields / icosphere lattices), "
                                  f"{tris} triangles, {total} AO rays (closest hit)",
@@ -546,6 +551,9 @@ def bench_c2(args, cw, W, dev) -> dict:
         "plain_ms": round(ms_plain, 4),
         "plain_mrays_s": round(total / ms_plain / 1e3, 2),
         "batched_untuned_ms": round(ms_untuned, 4),
+        "batched_untuned_roofline": roofline(
+            "cwbvh_trace<closest, MULTI> (persistent batch, library default configuration)", total, ms_untuned,
+            scene_bytes, bytes_c, traffic=rec.get("hbm_bytes_per_launch") if rec else None),
         "roofline": roofline("cwbvh_trace<closest, MULTI> (autotuned persistent batch, graph replay)", total,
                              ms_batch, scene_bytes, bytes_c,
                              {"nodes": nodes_c / total, "triangles": tris_c / total, "bytes": bytes_c / total}),
@@ -986,6 +994,8 @@ def profile_launches(args, cw, W) -> dict:
     byte counters), the three VALU issue probes (calibrate the issue
     ceiling), then --launches CWBVH and --launches BVH2 launches over the
     workload's rays with each scene's default configuration."""
+    if args.workload == "c2":
+        return profile_launches_c2(args, cw, W)
     if args.workload == "c0":
         wl = W.c0()
     elif args.workload == "c3":
@@ -1022,6 +1032,38 @@ def profile_launches(args, cw, W) -> dict:
             "cwbvh_config": cfg.as_dict(), "bvh2_config": cfg_b2.as_dict(), "node_bytes": 16 * s_cw.node_stride,
             "triangles": wl.mesh.num_triangles,
             "scene_bytes": {"cwbvh": s_cw.device_bytes(), "bvh2": s_b2.device_bytes()}}
+
+
+def profile_launches_c2(args, cw, W) -> dict:
+    """C2 launch sequence for rocprofv3: the 1 GiB copy, the three VALU
+    probes, then --launches launches of one batch over C2's 256 scenes with
+    the library's default batch configuration."""
+    built = build_c2(args, cw, W)
+    scenes = [cw.Scene(lay) for _, _, lay, _ in built]
+    pools = [cw.DeviceBuffer.from_numpy(rays) for _, _, _, rays in built]
+    sizes = [len(rays) for _, _, _, rays in built]
+    hits = [cw.DeviceBuffer(16 * n) for n in sizes]
+    items = [(sc, p.ptr, h.ptr, n) for sc, p, h, n in zip(scenes, pools, hits, sizes)]
+    batch = cw.Batch(items, cw.TRACE_CLOSEST, None)
+    copy_bytes = args.copy_mib << 20
+    src, dst = cw.DeviceBuffer(copy_bytes), cw.DeviceBuffer(copy_bytes)
+    src.zero()
+    cw.synchronize()
+    cw.profiler_start()
+    cw.check(cw.lib.cwbvh_copy_u4(dst.ptr, src.ptr, copy_bytes // 16, None))
+    probe_blocks = 8192
+    probe = cw.DeviceBuffer(4 * 256 * probe_blocks)
+    for mode in (0, 1, 2):
+        cw.check(cw.lib.cwbvh_valu_probe(mode, args.probe_iters, probe_blocks, probe.ptr, None))
+    for _ in range(args.launches):
+        batch.launch()
+    cw.synchronize()
+    cw.profiler_stop()
+    return {"workload": "C2", "rays": int(sum(sizes)), "scenes": len(items), "batch": "library default",
+            "copy_bytes": copy_bytes, "launches": args.launches, "flags": cw.TRACE_CLOSEST,
+            "probe": {"blocks": probe_blocks, "threads_per_block": 256, "iters": args.probe_iters},
+            "triangles": int(sum(b[1] for b in built)),
+            "scene_bytes": int(sum(sc.device_bytes() for sc in scenes))}
 
 
 # ------------------------------------------------------------ orchestrator
@@ -1116,7 +1158,7 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=2.0, help="target wall seconds of the CPU baseline sample")
     ap.add_argument("--node-stride", type=int, default=5, choices=[5, 8])
     # --config profile
-    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c3"])
+    ap.add_argument("--workload", default="c1", choices=["c0", "c1", "c2", "c3"])
     ap.add_argument("--launches", type=int, default=2)
     ap.add_argument("--copy-mib", type=int, default=1024)
     ap.add_argument("--probe-iters", type=int, default=4096)
